@@ -1,0 +1,158 @@
+/*
+ * nbx.h - C ABI of the MI355X Barnes-Hut gravity + KDK time-step engine.
+ *
+ * This is the drop-in boundary for NEXT's gravity and time-step path.  NEXT
+ * has no FFI; its path is two header-only C++ calls made by the driver loop
+ * (begrun.cpp:96-99):
+ *
+ *     real computeAdaptiveDt(const Particle&, real base_dt);   adaptive.h:23
+ *     void Step(ParticleSystem&, real dt);                     step.h:25
+ *
+ * on a caller-owned structure of arrays (particle.h:22-52).  Here the state
+ * lives on the GPU between calls: nbx_upload copies the caller's arrays in,
+ * nbx_adaptive_dt / nbx_step / nbx_run replace the two calls, and
+ * nbx_download copies positions and velocities back (for snapshots).
+ *
+ * Conventions
+ *  - Every function returns nbx_status (0 = OK, negative = error) unless
+ *    stated.  No C++ exception crosses this boundary.  The text of the last
+ *    error is kept per context (nbx_last_error).
+ *  - Arrays are plain host pointers owned by the caller; they are copied.
+ *    Particle arrays are in the caller's order (the order of the IC file);
+ *    the engine reorders bodies internally along its tree and always
+ *    returns results in the caller's order.
+ *  - A context owns its device buffers and one HIP stream.  A context is
+ *    used from one host thread at a time; several contexts may coexist.
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nbx_status {
+  NBX_OK = 0,
+  NBX_ERR_ARG = -1,        /* bad argument: null pointer, n < 0, theta < 0 ... */
+  NBX_ERR_NONFINITE = -2,  /* a position is NaN or infinite */
+  NBX_ERR_COINCIDENT = -3, /* two bodies share all 42 octree digits (NEXT's
+                              insertion recurses without end, octree.h:69-87) */
+  NBX_ERR_NOMEM = -4,      /* device allocation failed or budget exceeded */
+  NBX_ERR_HIP = -5,        /* HIP runtime error */
+  NBX_ERR_NODEVICE = -6,   /* no usable GPU */
+  NBX_ERR_STATE = -7       /* call out of order (e.g. no particles uploaded) */
+} nbx_status;
+
+typedef enum nbx_mode {
+  /* NEXT's algorithm in FP64: one body per leaf, one-level softened
+   * quadrupole, per-interaction adaptive softening, per-target MAC
+   * s/d < theta (octree.h:25-179, softening.h:21-35). */
+  NBX_MODE_REFERENCE = 0,
+  /* Bucket leaves, traceless quadrupole with parallel-axis propagation,
+   * Plummer softening eps, group MAC; FP32 or FP64 walk. */
+  NBX_MODE_PHYSICS = 1
+} nbx_mode;
+
+typedef enum nbx_precision {
+  NBX_PREC_FP64 = 0,   /* FP64 walk */
+  NBX_PREC_MIXED = 1   /* FP32 walk and accumulation; FP64 multipoles and state */
+} nbx_precision;
+
+typedef struct nbx_config {
+  int device;        /* HIP device ordinal */
+  int mode;          /* nbx_mode */
+  int precision;     /* nbx_precision (NBX_MODE_PHYSICS only) */
+  double theta;      /* opening angle; NEXT fixes 0.5 (step.h:211) */
+  double eps;        /* Plummer softening length (NBX_MODE_PHYSICS only) */
+  int leaf_size;     /* max bodies per leaf (NBX_MODE_PHYSICS only) */
+  int group_size;    /* targets per walk group (NBX_MODE_PHYSICS only) */
+} nbx_config;
+
+typedef struct nbx_ctx nbx_ctx;
+
+/* Per-phase device times (ms), summed over the steps / tree builds since
+ * the last nbx_reset_timing, measured with HIP events on the context's
+ * stream (profiling bit 1), and walk interaction counters (bit 2). */
+typedef struct nbx_timing {
+  int64_t steps;       /* steps (or accel evaluations) accumulated */
+  double keys_ms;      /* bounding box + Morton keys */
+  double sort_ms;      /* radix sort + reorder of the state */
+  double build_ms;     /* node layout from sorted keys */
+  double upward_ms;    /* multipole pass */
+  double walk_ms;      /* tree walk (+ fused closing half kick) */
+  double kick_ms;      /* opening half kick + drift */
+  double total_ms;     /* whole step */
+  int64_t nodes;       /* tree nodes of the last build */
+  int32_t depth;       /* deepest leaf of the last build */
+  int64_t n_pp;        /* particle-particle interactions (if counting) */
+  int64_t n_pc;        /* particle-cell interactions (if counting) */
+} nbx_timing;
+
+/* Fill *cfg with NEXT's settings: device 0, reference mode, FP64, theta 0.5. */
+void nbx_config_default(nbx_config* cfg);
+
+nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg);
+void nbx_destroy(nbx_ctx* ctx);
+const char* nbx_last_error(const nbx_ctx* ctx);
+nbx_status nbx_set_theta(nbx_ctx* ctx, double theta);
+
+/* Copy n particles in (Particle SoA of particle.h:22-52).  n may be 0. */
+nbx_status nbx_upload(nbx_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                      const double* vx, const double* vy, const double* vz, const double* m,
+                      const int32_t* type);
+
+/* Copy positions and velocities out, in upload order.  Any pointer may be
+ * null to skip that array. */
+nbx_status nbx_download(nbx_ctx* ctx, double* x, double* y, double* z, double* vx, double* vy,
+                        double* vz);
+
+int64_t nbx_size(const nbx_ctx* ctx);
+
+/* = computeAdaptiveDt (adaptive.h:23-52) on the device state. */
+nbx_status nbx_adaptive_dt(nbx_ctx* ctx, double base_dt, double* dt_out);
+
+/* = Step (step.h:204-383): one kick-drift-kick step of size dt. */
+nbx_status nbx_step(nbx_ctx* ctx, double dt);
+
+/* nsteps iterations of the driver loop body (begrun.cpp:97-99): dt from
+ * the current speeds, then one step, without host round trips.  dts_out
+ * (may be null) receives the nsteps time steps. */
+nbx_status nbx_run(nbx_ctx* ctx, double base_dt, int32_t nsteps, double* dts_out);
+
+/* Accelerations at the current positions (bhAccel over all targets,
+ * octree.h:133-179), upload order. */
+nbx_status nbx_accel(nbx_ctx* ctx, double* ax, double* ay, double* az);
+
+/* Accumulated timings and counters. */
+nbx_status nbx_get_timing(nbx_ctx* ctx, nbx_timing* out);
+nbx_status nbx_reset_timing(nbx_ctx* ctx);
+/* flags: 1 = record per-phase HIP events, 2 = count interactions. */
+nbx_status nbx_set_profiling(nbx_ctx* ctx, int flags);
+
+/* ---- parity hooks: the tree of the current positions ------------------ */
+
+/* Root centre and half-width (step.h:246-279) and each body's octree
+ * digits (octree.h:49-64): digits 0..20 in hi, 21..41 in lo, 3 bits each,
+ * first digit most significant; upload order. */
+nbx_status nbx_debug_keys(nbx_ctx* ctx, double* root4, uint64_t* hi, uint64_t* lo);
+
+/* Number of tree nodes of the current positions. */
+nbx_status nbx_debug_tree_size(nbx_ctx* ctx, int64_t* nodes);
+
+/* The node table in depth-first preorder, children in digit order 0..7
+ * (the order of NEXT's recursion), with: depth, leaf flag, body index
+ * (upload order, -1 for internal nodes), child-digit mask, geometric
+ * centre and half-width (geo4), mass, centre of mass (com3), quadrupole
+ * (q6: xx yy zz xy xz yz), and perm = bodies in leaf order. */
+nbx_status nbx_debug_tree(nbx_ctx* ctx, int32_t* depth, int32_t* leaf, int32_t* body,
+                          int32_t* cmask, double* geo4, double* mass, double* com3, double* q6,
+                          int32_t* perm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBX_H */

@@ -1,0 +1,105 @@
+// Shared device-side definitions for the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nbx {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kKeyDigits = 21;     // octree digits per 64-bit key word
+constexpr int kMaxLevels = 42;     // two key words
+constexpr int kMaxDepthSlots = 64; // histogram slots for node depths
+
+// One tree node in depth-first preorder, laid out for the walk: a wave reads
+// one node at a time at a wave-uniform index, so the record is fetched
+// through the scalar cache and broadcast to all lanes.
+struct alignas(16) RefNode {
+  double cx, cy, cz, m;    // centre of mass, mass
+  double size;             // half-width: root half-width * 2^-depth
+  double esum;             // size*0.015 + cbrt(m)*0.002 (softening numerator)
+  int32_t skip;            // preorder index of the first node after this subtree
+  int32_t body;            // leaf: sorted body index; internal: -1
+  int32_t depth;
+  int32_t first;           // first body (sorted index) of the subtree
+  double qxx, qyy, qzz, qxy, qxz, qyz;
+};
+static_assert(sizeof(RefNode) == 112, "RefNode layout");
+
+// Small per-context device block: geometry, time step, flags and the
+// per-level node counts the host needs to launch the upward pass.
+struct DevScalars {
+  double root[4];           // centre x, y, z, half-width
+  double dt;                // current time step
+  double base_dt;
+  unsigned long long v2max; // bit pattern of max |v|^2 (non-negative doubles order as integers)
+  int32_t err;              // bit 0: non-finite position
+  int32_t collisions;       // adjacent equal primary keys
+  int32_t coincident;       // adjacent equal primary and secondary keys
+  int32_t total_nodes;
+  int32_t internal_nodes;
+  int32_t max_leaf_depth;
+  int32_t level_count[kMaxDepthSlots];  // internal nodes per depth
+  unsigned long long n_pp, n_pc;        // interaction counters
+};
+
+// ---- wave primitives ------------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    int o = __shfl_xor(v, off, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive scan within a wave (Hillis-Steele over shuffles).
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t o = __shfl_up(v, off, kWave);
+    if (l >= off) v += o;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread across a block of BLOCK threads
+// (BLOCK a multiple of 64, at most 1024).  Returns the exclusive prefix;
+// *total receives the block sum.  `scratch` holds BLOCK/64 words.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* scratch, uint32_t* total) {
+  constexpr int W = BLOCK / kWave;
+  const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+  uint32_t inc = wave_incl_scan_u32(v);
+  if (l == kWave - 1) scratch[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    uint32_t s = l < W ? scratch[l] : 0u;
+    uint32_t si = wave_incl_scan_u32(s);
+    if (l < W) scratch[l] = si - s;
+    if (l == W - 1) scratch[W] = si;
+  }
+  __syncthreads();
+  uint32_t r = scratch[w] + inc - v;
+  *total = scratch[W];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace nbx

@@ -1,0 +1,587 @@
+// C ABI of the engine (include/nbx.h): context, device buffers and the
+// host-side orchestration of one KDK step on one HIP stream.
+//
+// One step (step.h:204-383, with the second tree of step n reused as the
+// first tree of step n+1: the walk is deterministic, so a(x_{n+1}) from the
+// closing kick of step n is bit-identical to the opening-kick force of
+// step n+1):
+//
+//   kick(a_n, dt/2) + drift(dt) + bbox partials     one fused kernel
+//   root, keys, radix sort, state gather            tree of x_{n+1}
+//   lcp/counts, scans  -> host reads node counts (one small copy)
+//   node emit, depth sorts, skip pairing, upward pass per depth
+//   walk + closing kick(a_{n+1}, dt/2) + max |v|^2  one fused kernel
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "nbx.h"
+#include "prims.hpp"
+#include "ref_kernels.hpp"
+
+using namespace nbx;
+
+namespace {
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+enum { EV_START, EV_KICK, EV_KEYS, EV_SORT, EV_BUILD, EV_UPWARD, EV_WALK, EV_COUNT };
+
+}  // namespace
+
+struct nbx_ctx {
+  nbx_config cfg{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t n = 0;
+  // state, double-buffered for the gather into key order
+  DBuf st[2][9];
+  int cur = 0;
+  DBuf acc[3];
+  DBuf key[2], idx[2], key2[2];
+  DBuf lcp, cnt_s, cnt_e;
+  DBuf nodes;
+  DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
+  DBuf scratch, bbox_part, dts;
+  DevScalars* sc = nullptr;   // device
+  DevScalars* hsc = nullptr;  // pinned host mirror
+  bool acc_valid = false;     // acc holds a(x) of the current positions
+  bool tree_valid = false;    // nodes hold the tree of the current positions
+  const uint64_t* skey = nullptr;
+  const uint64_t* skey2 = nullptr;
+  int64_t total_nodes = 0;
+  int32_t max_depth = 0;
+  int prof = 0;
+  hipEvent_t ev[EV_COUNT] = {};
+  nbx_timing tim{};
+};
+
+namespace {
+
+template <class T>
+T* P(const DBuf& b) { return static_cast<T*>(b.p); }
+
+nbx_status fail(nbx_ctx* c, nbx_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+#define HIPCHK(ctx, expr)                                                                        \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(ctx, NBX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define NBXCHK(expr)                          \
+  do {                                        \
+    nbx_status s_ = (expr);                   \
+    if (s_ != NBX_OK) return s_;              \
+  } while (0)
+
+// Grow a buffer to at least `bytes` (contents not kept).
+nbx_status ensure(nbx_ctx* c, DBuf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.p) return NBX_OK;
+  if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+  const size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, NBX_ERR_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  b.bytes = want;
+  return NBX_OK;
+}
+
+void release(DBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+StateArrays view(nbx_ctx* c, int which) {
+  DBuf* s = c->st[which];
+  return StateArrays{P<double>(s[0]), P<double>(s[1]), P<double>(s[2]), P<double>(s[3]), P<double>(s[4]),
+                     P<double>(s[5]), P<double>(s[6]), P<int32_t>(s[7]), P<int32_t>(s[8])};
+}
+
+void record(nbx_ctx* c, int e) {
+  if (c->prof & 1) (void)hipEventRecord(c->ev[e], c->stream);
+}
+
+nbx_status sync_scalars(nbx_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBX_OK;
+}
+
+// Buffers sized by the particle count.
+nbx_status ensure_particles(nbx_ctx* c, int64_t n) {
+  const size_t nn = (size_t)std::max<int64_t>(n, 1);
+  for (int w = 0; w < 2; ++w)
+    for (int k = 0; k < 9; ++k) NBXCHK(ensure(c, c->st[w][k], nn * (k < 7 ? 8 : 4)));
+  for (int k = 0; k < 3; ++k) NBXCHK(ensure(c, c->acc[k], nn * 8));
+  for (int w = 0; w < 2; ++w) {
+    NBXCHK(ensure(c, c->key[w], nn * 8));
+    NBXCHK(ensure(c, c->idx[w], nn * 4));
+  }
+  NBXCHK(ensure(c, c->lcp, nn * 4));
+  NBXCHK(ensure(c, c->cnt_s, nn * 4));
+  NBXCHK(ensure(c, c->cnt_e, nn * 4));
+  NBXCHK(ensure(c, c->bbox_part, (size_t)kBboxBlocks * 6 * 8));
+  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(n), scan_scratch_words(n)) * 4));
+  return NBX_OK;
+}
+
+// Tree of the current positions: keys, sort, gather, layout, multipoles.
+// `bbox_ready`: the root partials were produced by the fused kick/drift.
+nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
+  const int64_t n = c->n;
+  hipStream_t s = c->stream;
+  StateArrays S = view(c, c->cur);
+  if (!bbox_ready) launch_bbox(n, S, P<double>(c->bbox_part), c->sc, s);
+  launch_fill_scalars(c->sc, 1 | 4, s);
+  launch_keys(n, S, c->sc, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
+  record(c, EV_KEYS);
+  uint32_t* scr = P<uint32_t>(c->scratch);
+  const bool in1 = radix_sort_u64(P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key[1]),
+                                  P<uint32_t>(c->idx[1]), n, 63, scr, s);
+  const uint64_t* skey = P<uint64_t>(c->key[in1 ? 1 : 0]);
+  const uint32_t* perm = P<uint32_t>(c->idx[in1 ? 1 : 0]);
+  launch_gather_state(n, perm, S, view(c, 1 - c->cur), s);
+  c->cur = 1 - c->cur;
+  S = view(c, c->cur);
+  record(c, EV_SORT);
+
+  uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
+  uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
+  launch_lcp_count(n, skey, nullptr, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), c->sc, s);
+  scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
+  scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
+  NBXCHK(sync_scalars(c));
+  if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
+
+  const uint64_t* skey2 = nullptr;
+  if (c->hsc->collisions > 0) {
+    // Bodies sharing all 21 digits of the first key word: sort by
+    // (first word, second word).  The state is already in first-word order;
+    // a stable sort by the second word followed by a stable sort by the first
+    // gives the lexicographic order.
+    for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)n * 8));
+    const int fk = in1 ? 1 : 0;   // buffer holding the sorted first words
+    launch_keys2(n, S, c->sc, P<uint64_t>(c->key2[0]), s);
+    launch_iota(n, P<uint32_t>(c->idx[0]), s);
+    const bool a = radix_sort_u64(P<uint64_t>(c->key2[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key2[1]),
+                                  P<uint32_t>(c->idx[1]), n, 63, scr, s);
+    uint32_t* p1 = P<uint32_t>(c->idx[a ? 1 : 0]);
+    uint32_t* p1o = P<uint32_t>(c->idx[a ? 0 : 1]);
+    uint64_t* kfree = P<uint64_t>(c->key[1 - fk]);
+    launch_gather_u64(n, p1, P<uint64_t>(c->key[fk]), kfree, s);
+    const bool b = radix_sort_u64(kfree, p1, P<uint64_t>(c->key[fk]), p1o, n, 63, scr, s);
+    skey = b ? P<uint64_t>(c->key[fk]) : kfree;
+    const uint32_t* pf = b ? p1o : p1;
+    launch_gather_state(n, pf, S, view(c, 1 - c->cur), s);
+    c->cur = 1 - c->cur;
+    S = view(c, c->cur);
+    launch_keys2(n, S, c->sc, P<uint64_t>(c->key2[0]), s);
+    skey2 = P<uint64_t>(c->key2[0]);
+    launch_fill_scalars(c->sc, 1, s);
+    launch_lcp_count(n, skey, skey2, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), c->sc, s);
+    scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
+    scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
+    NBXCHK(sync_scalars(c));
+    if (c->hsc->coincident > 0)
+      return fail(c, NBX_ERR_COINCIDENT,
+                  std::to_string(c->hsc->coincident) + " pair(s) of bodies share all 42 octree digits");
+  }
+  c->skey = skey;
+  c->skey2 = skey2;
+
+  const int64_t total = (uint32_t)c->hsc->total_nodes;
+  const int64_t m = (uint32_t)c->hsc->internal_nodes;
+  c->total_nodes = total;
+  c->max_depth = c->hsc->max_leaf_depth;
+  NBXCHK(ensure(c, c->nodes, (size_t)total * sizeof(RefNode)));
+  const size_t mm = (size_t)std::max<int64_t>(m, 1) * 4;
+  for (DBuf* b : {&c->st_key, &c->st_val, &c->en_key, &c->en_val, &c->st_key2, &c->st_val2, &c->en_key2,
+                  &c->en_val2, &c->level_list})
+    NBXCHK(ensure(c, *b, mm));
+  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(std::max(n, m)), scan_scratch_words(n)) * 4));
+  scr = P<uint32_t>(c->scratch);
+  RefNode* nodes = P<RefNode>(c->nodes);
+  launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), S, c->sc, nodes,
+              P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), s);
+  const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
+                                 P<uint32_t>(c->st_val2), m, 8, scr, s);
+  const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
+                                 P<uint32_t>(c->en_val2), m, 8, scr, s);
+  launch_pair(m, P<uint32_t>(sa ? c->st_val2 : c->st_val), P<uint32_t>(ea ? c->en_val2 : c->en_val), nodes,
+              P<int32_t>(c->level_list), s);
+  record(c, EV_BUILD);
+  int64_t off[kMaxDepthSlots + 1];
+  off[0] = 0;
+  for (int d = 0; d < kMaxDepthSlots; ++d) off[d + 1] = off[d] + c->hsc->level_count[d];
+  for (int d = kMaxDepthSlots - 1; d >= 0; --d)
+    if (c->hsc->level_count[d] > 0) launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, s);
+  record(c, EV_UPWARD);
+  c->tree_valid = true;
+  return NBX_OK;
+}
+
+nbx_status walk(nbx_ctx* c, bool kick) {
+  const bool count = (c->prof & 2) != 0;
+  if (count) launch_fill_scalars(c->sc, 8, c->stream);
+  if (kick) launch_fill_scalars(c->sc, 2, c->stream);
+  launch_walk(c->n, c->total_nodes, P<RefNode>(c->nodes), view(c, c->cur), c->cfg.theta, P<double>(c->acc[0]),
+              P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
+  record(c, EV_WALK);
+  HIPCHK(c, hipGetLastError());
+  c->acc_valid = true;
+  return NBX_OK;
+}
+
+float elapsed(nbx_ctx* c, int a, int b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.f;
+  return ms;
+}
+
+nbx_status finish_timing(nbx_ctx* c, bool with_kick) {
+  if (c->prof & 1) {
+    HIPCHK(c, hipEventSynchronize(c->ev[EV_WALK]));
+    const int k0 = with_kick ? EV_KICK : EV_START;
+    if (with_kick) c->tim.kick_ms += elapsed(c, EV_START, EV_KICK);
+    c->tim.keys_ms += elapsed(c, k0, EV_KEYS);
+    c->tim.sort_ms += elapsed(c, EV_KEYS, EV_SORT);
+    c->tim.build_ms += elapsed(c, EV_SORT, EV_BUILD);
+    c->tim.upward_ms += elapsed(c, EV_BUILD, EV_UPWARD);
+    c->tim.walk_ms += elapsed(c, EV_UPWARD, EV_WALK);
+    c->tim.total_ms += elapsed(c, EV_START, EV_WALK);
+  }
+  if (c->prof & 2) {
+    HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->tim.n_pp += (int64_t)c->hsc->n_pp;
+    c->tim.n_pc += (int64_t)c->hsc->n_pc;
+  }
+  c->tim.steps += 1;
+  c->tim.nodes = c->total_nodes;
+  c->tim.depth = c->max_depth;
+  return NBX_OK;
+}
+
+// a(x) of the current positions, without a kick.
+nbx_status ensure_accel(nbx_ctx* c) {
+  if (c->acc_valid) return NBX_OK;
+  record(c, EV_START);
+  NBXCHK(build_tree(c, false));
+  NBXCHK(walk(c, false));
+  return NBX_OK;
+}
+
+// One KDK step with the dt held in the device scalars.
+nbx_status step_device(nbx_ctx* c) {
+  if (c->n == 0) return NBX_OK;
+  NBXCHK(ensure_accel(c));
+  record(c, EV_START);
+  launch_kick_drift(c->n, view(c, c->cur), P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc,
+                    P<double>(c->bbox_part), c->stream);
+  launch_bbox_final(P<double>(c->bbox_part),
+                    (int)std::min<int64_t>(std::max<int64_t>(div_up(c->n, kBlock), 1), kBboxBlocks), c->sc,
+                    c->stream);
+  c->acc_valid = false;
+  c->tree_valid = false;
+  record(c, EV_KICK);
+  NBXCHK(build_tree(c, true));
+  NBXCHK(walk(c, true));
+  return finish_timing(c, true);
+}
+
+nbx_status check_ctx(nbx_ctx* c) {
+  if (!c) return NBX_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->cfg.device));
+  return NBX_OK;
+}
+
+void unpermute(const std::vector<int32_t>& id, const double* src, double* dst) {
+  for (size_t k = 0; k < id.size(); ++k) dst[id[k]] = src[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+void nbx_config_default(nbx_config* cfg) {
+  if (!cfg) return;
+  cfg->device = 0;
+  cfg->mode = NBX_MODE_REFERENCE;
+  cfg->precision = NBX_PREC_FP64;
+  cfg->theta = 0.5;
+  cfg->eps = 0.01;
+  cfg->leaf_size = 16;
+  cfg->group_size = 64;
+}
+
+nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
+  if (!out) return NBX_ERR_ARG;
+  *out = nullptr;
+  nbx_config c0;
+  nbx_config_default(&c0);
+  if (cfg) c0 = *cfg;
+  if (!(c0.theta >= 0.0)) return NBX_ERR_ARG;
+  if (c0.mode != NBX_MODE_REFERENCE) return NBX_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return NBX_ERR_NODEVICE;
+  }
+  if (c0.device < 0 || c0.device >= ndev) return NBX_ERR_NODEVICE;
+  if (hipSetDevice(c0.device) != hipSuccess) return NBX_ERR_HIP;
+  nbx_ctx* c = new nbx_ctx;
+  c->cfg = c0;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->sc, sizeof(DevScalars)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->hsc), sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
+    nbx_destroy(c);
+    return NBX_ERR_HIP;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) { nbx_destroy(c); return NBX_ERR_HIP; }
+  std::memset(c->hsc, 0, sizeof(DevScalars));
+  if (hipMemcpyAsync(c->sc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    nbx_destroy(c);
+    return NBX_ERR_HIP;
+  }
+  *out = c;
+  return NBX_OK;
+}
+
+void nbx_destroy(nbx_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& w : c->st) for (auto& b : w) release(b);
+  for (auto& b : c->acc) release(b);
+  for (auto& b : c->key) release(b);
+  for (auto& b : c->idx) release(b);
+  for (auto& b : c->key2) release(b);
+  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
+                  &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->bbox_part,
+                  &c->dts})
+    release(*b);
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->sc) (void)hipFree(c->sc);
+  if (c->hsc) (void)hipHostFree(c->hsc);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* nbx_last_error(const nbx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+nbx_status nbx_set_theta(nbx_ctx* c, double theta) {
+  if (!c || !(theta >= 0.0)) return NBX_ERR_ARG;
+  c->cfg.theta = theta;
+  c->acc_valid = false;
+  return NBX_OK;
+}
+
+int64_t nbx_size(const nbx_ctx* c) { return c ? c->n : -1; }
+
+nbx_status nbx_upload(nbx_ctx* c, int64_t n, const double* x, const double* y, const double* z, const double* vx,
+                      const double* vy, const double* vz, const double* m, const int32_t* type) {
+  NBXCHK(check_ctx(c));
+  if (n < 0 || n > (int64_t)INT32_MAX / 64) return fail(c, NBX_ERR_ARG, "particle count out of range");
+  if (n > 0 && !(x && y && z && vx && vy && vz && m && type)) return fail(c, NBX_ERR_ARG, "null particle array");
+  NBXCHK(ensure_particles(c, n));
+  c->n = n;
+  c->cur = 0;
+  c->acc_valid = false;
+  c->tree_valid = false;
+  if (n > 0) {
+    const void* src[8] = {x, y, z, vx, vy, vz, m, type};
+    for (int k = 0; k < 8; ++k)
+      HIPCHK(c, hipMemcpyAsync(c->st[0][k].p, src[k], (size_t)n * (k < 7 ? 8 : 4), hipMemcpyHostToDevice, c->stream));
+    std::vector<int32_t> id((size_t)n);
+    for (int64_t i = 0; i < n; ++i) id[(size_t)i] = (int32_t)i;
+    HIPCHK(c, hipMemcpyAsync(c->st[0][8].p, id.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return NBX_OK;
+}
+
+nbx_status nbx_download(nbx_ctx* c, double* x, double* y, double* z, double* vx, double* vy, double* vz) {
+  NBXCHK(check_ctx(c));
+  const int64_t n = c->n;
+  if (n == 0) return NBX_OK;
+  std::vector<int32_t> id((size_t)n);
+  std::vector<double> tmp((size_t)n);
+  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur][8].p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double* outs[6] = {x, y, z, vx, vy, vz};
+  for (int k = 0; k < 6; ++k) {
+    if (!outs[k]) continue;
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->st[c->cur][k].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpermute(id, tmp.data(), outs[k]);
+  }
+  return NBX_OK;
+}
+
+nbx_status nbx_adaptive_dt(nbx_ctx* c, double base_dt, double* dt_out) {
+  NBXCHK(check_ctx(c));
+  if (!dt_out) return fail(c, NBX_ERR_ARG, "null dt_out");
+  launch_fill_scalars(c->sc, 2, c->stream);
+  if (c->n > 0) launch_maxv2(c->n, view(c, c->cur), c->sc, c->stream);
+  launch_dt(c->sc, base_dt, nullptr, 0, c->stream);
+  NBXCHK(sync_scalars(c));
+  *dt_out = c->hsc->dt;
+  return NBX_OK;
+}
+
+nbx_status nbx_step(nbx_ctx* c, double dt) {
+  NBXCHK(check_ctx(c));
+  launch_set_dt(c->sc, dt, c->stream);
+  NBXCHK(step_device(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBX_OK;
+}
+
+nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) {
+  NBXCHK(check_ctx(c));
+  if (nsteps < 0) return fail(c, NBX_ERR_ARG, "negative step count");
+  if (nsteps == 0) return NBX_OK;
+  if (c->n == 0) {  // computeAdaptiveDt of no particles is base_dt; Step is a no-op
+    if (dts_out) for (int s = 0; s < nsteps; ++s) dts_out[s] = base_dt;
+    return NBX_OK;
+  }
+  NBXCHK(ensure(c, c->dts, (size_t)nsteps * 8));
+  launch_fill_scalars(c->sc, 2, c->stream);
+  launch_maxv2(c->n, view(c, c->cur), c->sc, c->stream);
+  for (int32_t s = 0; s < nsteps; ++s) {
+    launch_dt(c->sc, base_dt, P<double>(c->dts), s, c->stream);
+    NBXCHK(step_device(c));
+  }
+  if (dts_out)
+    HIPCHK(c, hipMemcpyAsync(dts_out, c->dts.p, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBX_OK;
+}
+
+nbx_status nbx_accel(nbx_ctx* c, double* ax, double* ay, double* az) {
+  NBXCHK(check_ctx(c));
+  if (c->n == 0) return NBX_OK;
+  if (!(ax && ay && az)) return fail(c, NBX_ERR_ARG, "null output");
+  const bool fresh = !c->acc_valid;
+  NBXCHK(ensure_accel(c));
+  if (fresh) NBXCHK(finish_timing(c, false));
+  const int64_t n = c->n;
+  std::vector<int32_t> id((size_t)n);
+  std::vector<double> tmp((size_t)n);
+  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur][8].p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double* outs[3] = {ax, ay, az};
+  for (int k = 0; k < 3; ++k) {
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->acc[k].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpermute(id, tmp.data(), outs[k]);
+  }
+  return NBX_OK;
+}
+
+nbx_status nbx_get_timing(nbx_ctx* c, nbx_timing* out) {
+  if (!c || !out) return NBX_ERR_ARG;
+  *out = c->tim;
+  return NBX_OK;
+}
+
+nbx_status nbx_reset_timing(nbx_ctx* c) {
+  if (!c) return NBX_ERR_ARG;
+  c->tim = nbx_timing{};
+  return NBX_OK;
+}
+
+nbx_status nbx_set_profiling(nbx_ctx* c, int flags) {
+  if (!c) return NBX_ERR_ARG;
+  c->prof = flags;
+  return NBX_OK;
+}
+
+nbx_status nbx_debug_keys(nbx_ctx* c, double* root4, uint64_t* hi, uint64_t* lo) {
+  NBXCHK(check_ctx(c));
+  const int64_t n = c->n;
+  if (!root4) return fail(c, NBX_ERR_ARG, "null root4");
+  launch_bbox(n, view(c, c->cur), P<double>(c->bbox_part), c->sc, c->stream);
+  NBXCHK(sync_scalars(c));
+  std::memcpy(root4, c->hsc->root, 4 * sizeof(double));
+  if (n == 0) return NBX_OK;
+  if (!(hi && lo)) return fail(c, NBX_ERR_ARG, "null key output");
+  for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)n * 8));
+  launch_keys_export(n, view(c, c->cur), c->sc, P<uint64_t>(c->key2[0]), P<uint64_t>(c->key2[1]), c->stream);
+  HIPCHK(c, hipMemcpyAsync(hi, c->key2[0].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(lo, c->key2[1].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->skey2 = nullptr;  // the second-word buffers were reused
+  c->tree_valid = false;
+  return NBX_OK;
+}
+
+nbx_status nbx_debug_tree_size(nbx_ctx* c, int64_t* nodes) {
+  NBXCHK(check_ctx(c));
+  if (!nodes) return fail(c, NBX_ERR_ARG, "null output");
+  if (c->n == 0) { *nodes = 0; return NBX_OK; }
+  if (!c->tree_valid) {
+    NBXCHK(build_tree(c, false));
+    c->acc_valid = false;
+  }
+  *nodes = c->total_nodes;
+  return NBX_OK;
+}
+
+nbx_status nbx_debug_tree(nbx_ctx* c, int32_t* depth, int32_t* leaf, int32_t* body, int32_t* cmask, double* geo4,
+                          double* mass, double* com3, double* q6, int32_t* perm) {
+  int64_t total = 0;
+  NBXCHK(nbx_debug_tree_size(c, &total));
+  if (total == 0) return NBX_OK;
+  if (!(depth && leaf && body && cmask && geo4 && mass && com3 && q6 && perm))
+    return fail(c, NBX_ERR_ARG, "null output");
+  // one staging allocation for all outputs
+  const size_t ni = (size_t)total * 4, nd = (size_t)total * 8;
+  DBuf tmp;
+  NBXCHK(ensure(c, tmp, 4 * ni + 14 * nd));
+  char* b = P<char>(tmp);
+  int32_t* d_depth = reinterpret_cast<int32_t*>(b);
+  int32_t* d_leaf = d_depth + total;
+  int32_t* d_body = d_leaf + total;
+  int32_t* d_cmask = d_body + total;
+  double* d_geo = reinterpret_cast<double*>(b + 4 * ni);
+  double* d_mass = d_geo + 4 * total;
+  double* d_com = d_mass + total;
+  double* d_q = d_com + 3 * total;
+  launch_export(total, P<RefNode>(c->nodes), c->skey, c->skey2, P<int32_t>(c->st[c->cur][8]), c->sc, d_depth, d_leaf,
+                d_body, d_cmask, d_geo, d_mass, d_com, d_q, c->stream);
+  hipError_t e = hipMemcpyAsync(depth, d_depth, ni, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(leaf, d_leaf, ni, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(body, d_body, ni, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(cmask, d_cmask, ni, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(geo4, d_geo, 4 * nd, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(mass, d_mass, nd, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(com3, d_com, 3 * nd, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(q6, d_q, 6 * nd, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(perm, c->st[c->cur][8].p, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  release(tmp);
+  if (e != hipSuccess) return fail(c, NBX_ERR_HIP, std::string("tree export: ") + hipGetErrorString(e));
+  return NBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,636 @@
+// Reference-mode kernels: NEXT's FP64 gravity + time-step path on gfx950.
+//
+// Compiled with -ffp-contract=off: every product and sum is rounded as
+// written, in the order NEXT writes it, so the GPU reproduces the reference
+// arithmetic operation by operation.
+//
+// Pipeline of one tree build (positions in `st`):
+//   bbox -> root cube (step.h:246-279)
+//   keys: each body replays the FP64 node-centre recurrence of
+//         octree.h:49-64 for 21 levels -> 63-bit digit key
+//   stable radix sort of keys, gather of the state into key order
+//   lcp of neighbouring keys -> the one-body-per-leaf octree of
+//         octree.h:69-87 in depth-first preorder (children 0..7), built from
+//         the rule "an internal node exists for every digit prefix shared by
+//         two or more bodies", with skip pointers for stackless traversal
+//   upward pass, one launch per depth, deepest first (octree.h:92-127)
+//   walk: one wave per 64 consecutive targets; the wave visits the union of
+//         its lanes' depth-first paths in preorder, each lane applying its own
+//         acceptance test (octree.h:133-179), so every lane accumulates the
+//         same terms in the same order as the recursive reference.
+#include "common.hpp"
+#include "prims.hpp"
+#include "ref_kernels.hpp"
+
+#include <algorithm>
+#include <climits>
+
+namespace nbx {
+namespace {
+
+__device__ __forceinline__ void lo_upd(double& lo, double p) { if (p < lo) lo = p; }   // std::min(lo, p)
+__device__ __forceinline__ void hi_upd(double& hi, double p) { if (hi < p) hi = p; }   // std::max(hi, p)
+
+// Block-wide min/max of 6 values (3 lows, 3 highs) written as one partial.
+__device__ void block_bbox_write(double lo[3], double hi[3], double* part) {
+  __shared__ double sm[kBlock / kWave][6];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo_upd(lo[k], __shfl_xor(lo[k], off, kWave));
+      hi_upd(hi[k], __shfl_xor(hi[k], off, kWave));
+    }
+  }
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    for (int k = 0; k < 3; ++k) { sm[w][k] = lo[k]; sm[w][3 + k] = hi[k]; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kBlock / kWave; ++q)
+      for (int k = 0; k < 3; ++k) { lo_upd(lo[k], sm[q][k]); hi_upd(hi[k], sm[q][3 + k]); }
+    for (int k = 0; k < 3; ++k) { part[6 * blockIdx.x + k] = lo[k]; part[6 * blockIdx.x + 3 + k] = hi[k]; }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int what) {
+  const int t = threadIdx.x;
+  if (what & 1) {
+    if (t < kMaxDepthSlots) sc->level_count[t] = 0;
+    if (t == 0) { sc->collisions = 0; sc->coincident = 0; sc->max_leaf_depth = 0; }
+  }
+  if ((what & 2) && t == 0) sc->v2max = 0ull;
+  if ((what & 4) && t == 0) sc->err = 0;
+  if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; }
+}
+
+__global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
+
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z, double* part) {
+  double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double p[3] = {x[i], y[i], z[i]};
+    for (int k = 0; k < 3; ++k) { lo_upd(lo[k], p[k]); hi_upd(hi[k], p[k]); }
+  }
+  block_bbox_write(lo, hi, part);
+}
+
+// Root centre and half-width from the partials (step.h:272-279).
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const double* __restrict__ part, int nparts, DevScalars* sc) {
+  double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+  for (int b = threadIdx.x; b < nparts; b += kBlock)
+    for (int k = 0; k < 3; ++k) { lo_upd(lo[k], part[6 * b + k]); hi_upd(hi[k], part[6 * b + 3 + k]); }
+  __shared__ double fin[6];
+  __syncthreads();
+  // reuse the block reducer into a one-entry partial
+  block_bbox_write(lo, hi, fin - 6 * (int64_t)blockIdx.x);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double cx = (fin[0] + fin[3]) * 0.5;
+    const double cy = (fin[1] + fin[4]) * 0.5;
+    const double cz = (fin[2] + fin[5]) * 0.5;
+    double s = fin[3] - fin[0];
+    const double ey = fin[4] - fin[1], ez = fin[5] - fin[2];
+    if (s < ey) s = ey;
+    if (s < ez) s = ez;
+    s = s * 0.5;
+    if (s <= 0.0) s = 1.0;
+    sc->root[0] = cx; sc->root[1] = cy; sc->root[2] = cz; sc->root[3] = s;
+  }
+}
+
+// Digits of one body along the node-centre recurrence (octree.h:49-64),
+// levels [l0, l1), starting from the root.
+__device__ __forceinline__ void key_digits(double px, double py, double pz, const double* root, int l0, int l1,
+                                           uint64_t& hi, uint64_t& lo) {
+  double cx = root[0], cy = root[1], cz = root[2], s = root[3];
+  hi = 0; lo = 0;
+  for (int l = 0; l < l1; ++l) {
+    const uint32_t o = (px > cx ? 1u : 0u) | (py > cy ? 2u : 0u) | (pz > cz ? 4u : 0u);
+    if (l >= l0) {
+      if (l < kKeyDigits) hi = (hi << 3) | o; else lo = (lo << 3) | o;
+    }
+    const double hs = s * 0.5;
+    cx = cx + ((o & 1u) ? hs : -hs);
+    cy = cy + ((o & 2u) ? hs : -hs);
+    cz = cz + ((o & 4u) ? hs : -hs);
+    s = hs;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_keys(int64_t n, StateArrays st, const DevScalars* __restrict__ sc,
+                                                 uint64_t* __restrict__ key, uint32_t* __restrict__ idx,
+                                                 DevScalars* scw) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double px = st.x[i], py = st.y[i], pz = st.z[i];
+  if (!(__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(pz))) atomicOr(&scw->err, 1);
+  double root[4] = {sc->root[0], sc->root[1], sc->root[2], sc->root[3]};
+  uint64_t hi, lo;
+  key_digits(px, py, pz, root, 0, kKeyDigits, hi, lo);
+  key[i] = hi;
+  idx[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_keys2(int64_t n, StateArrays st, const DevScalars* __restrict__ sc,
+                                                  uint64_t* __restrict__ key2) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double root[4] = {sc->root[0], sc->root[1], sc->root[2], sc->root[3]};
+  uint64_t hi, lo;
+  key_digits(st.x[i], st.y[i], st.z[i], root, kKeyDigits, kMaxLevels, hi, lo);
+  key2[i] = lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_keys_export(int64_t n, StateArrays st, const DevScalars* __restrict__ sc,
+                                                        uint64_t* __restrict__ hi_out,
+                                                        uint64_t* __restrict__ lo_out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double root[4] = {sc->root[0], sc->root[1], sc->root[2], sc->root[3]};
+  uint64_t hi, lo;
+  key_digits(st.x[i], st.y[i], st.z[i], root, 0, kMaxLevels, hi, lo);
+  const int32_t o = st.id[i];
+  hi_out[o] = hi;
+  lo_out[o] = lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(int64_t n, uint32_t* idx) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) idx[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_u64(int64_t n, const uint32_t* __restrict__ perm,
+                                                       const uint64_t* __restrict__ src, uint64_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_state(int64_t n, const uint32_t* __restrict__ perm, StateArrays a,
+                                                         StateArrays b) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = perm[i];
+  b.x[i] = a.x[p]; b.y[i] = a.y[p]; b.z[i] = a.z[p];
+  b.vx[i] = a.vx[p]; b.vy[i] = a.vy[p]; b.vz[i] = a.vz[p];
+  b.m[i] = a.m[p]; b.type[i] = a.type[p]; b.id[i] = a.id[p];
+}
+
+// Number of leading octree digits two keys share (0..21 per word).
+__device__ __forceinline__ int lcp_word(uint64_t a, uint64_t b) {
+  const uint64_t d = a ^ b;
+  return d ? (__clzll((long long)d) - 1) / 3 : kKeyDigits;
+}
+
+__device__ __forceinline__ int lcp_pair(const uint64_t* key, const uint64_t* key2, int64_t i, int& equal) {
+  int l = lcp_word(key[i], key[i + 1]);
+  equal = 0;
+  if (l == kKeyDigits) {
+    if (key2) {
+      l += lcp_word(key2[i], key2[i + 1]);
+      if (l == kMaxLevels) equal = 1;
+    } else {
+      equal = 1;
+    }
+  }
+  return l;
+}
+
+// lcp[i] = digits shared by sorted bodies i and i+1 (-1 past the end).
+// Body i starts the internal nodes at depths (lcp[i-1], lcp[i]] and its leaf;
+// it ends the internal nodes at depths (lcp[i], lcp[i-1]].
+__global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t* __restrict__ key,
+                                                      const uint64_t* __restrict__ key2, int32_t* __restrict__ lcp,
+                                                      uint32_t* __restrict__ cnt_s, uint32_t* __restrict__ cnt_e,
+                                                      DevScalars* sc) {
+  __shared__ uint32_t hist[kMaxDepthSlots];
+  __shared__ int32_t smax;
+  __shared__ uint32_t scoll;
+  if (threadIdx.x < kMaxDepthSlots) hist[threadIdx.x] = 0;
+  if (threadIdx.x == 0) { smax = 0; scoll = 0; }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    int eq_i = 0, eq_p = 0;
+    const int li = (i + 1 < n) ? lcp_pair(key, key2, i, eq_i) : -1;
+    const int lp = (i > 0) ? lcp_pair(key, key2, i - 1, eq_p) : -1;
+    lcp[i] = li;
+    cnt_s[i] = 1u + (uint32_t)(li > lp ? li - lp : 0);
+    cnt_e[i] = (uint32_t)(lp > li ? lp - li : 0);
+    for (int d = lp + 1; d <= li; ++d) atomicAdd(&hist[d], 1u);
+    atomicMax(&smax, 1 + (li > lp ? li : lp));
+    if (eq_i) atomicAdd(&scoll, 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxDepthSlots && hist[threadIdx.x]) atomicAdd(&sc->level_count[threadIdx.x], (int)hist[threadIdx.x]);
+  if (threadIdx.x == 0) {
+    atomicMax(&sc->max_leaf_depth, smax);
+    if (scoll) atomicAdd(key2 ? &sc->coincident : &sc->collisions, (int)scoll);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __restrict__ lcp,
+                                                 const uint32_t* __restrict__ pre_off,
+                                                 const uint32_t* __restrict__ end_off, StateArrays st,
+                                                 const DevScalars* __restrict__ sc, RefNode* __restrict__ nodes,
+                                                 uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
+                                                 uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int li = lcp[i];
+  const int lp = i > 0 ? lcp[i - 1] : -1;
+  const uint32_t pre = pre_off[i];
+  const double s0 = sc->root[3];
+  const int ns = li > lp ? li - lp : 0;
+  const uint32_t sbase = pre - (uint32_t)i;
+  for (int t = 0; t < ns; ++t) {
+    const int d = lp + 1 + t;
+    const uint32_t p = pre + t;
+    st_key[sbase + t] = (uint32_t)d;
+    st_val[sbase + t] = p;
+    RefNode nd;
+    nd.cx = nd.cy = nd.cz = nd.m = 0.0;
+    nd.size = ldexp(s0, -d);
+    nd.esum = 0.0;
+    nd.skip = 0;
+    nd.body = -1;
+    nd.depth = d;
+    nd.first = (int32_t)i;
+    nd.qxx = nd.qyy = nd.qzz = nd.qxy = nd.qxz = nd.qyz = 0.0;
+    nodes[p] = nd;
+  }
+  {
+    const uint32_t p = pre + ns;
+    const int d = 1 + (li > lp ? li : lp);
+    RefNode nd;
+    nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
+    nd.m = st.m[i];
+    nd.size = ldexp(s0, -d);
+    // softening.h:26-27: size*0.015 + cbrt(m)*0.002, summed before the taper
+    nd.esum = nd.size * 0.015 + cbrt(nd.m) * 0.002;
+    nd.skip = (int32_t)(p + 1);
+    nd.body = (int32_t)i;
+    nd.depth = d;
+    nd.first = (int32_t)i;
+    nd.qxx = nd.qyy = nd.qzz = nd.qxy = nd.qxz = nd.qyz = 0.0;
+    nodes[p] = nd;
+  }
+  const int ne = lp > li ? lp - li : 0;
+  if (ne) {
+    const uint32_t skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
+    const uint32_t eo = end_off[i];
+    for (int t = 0; t < ne; ++t) {
+      en_key[eo + t] = (uint32_t)(li + 1 + t);
+      en_val[eo + t] = skipv;
+    }
+  }
+}
+
+// The k-th internal node of depth d in start order is the k-th in end order.
+__global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const uint32_t* __restrict__ st_val,
+                                                 const uint32_t* __restrict__ en_val, RefNode* __restrict__ nodes,
+                                                 int32_t* __restrict__ level_list) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t p = st_val[k];
+  nodes[p].skip = (int32_t)en_val[k];
+  level_list[k] = (int32_t)p;
+}
+
+// Mass, centre of mass and one-level softened quadrupole of the internal
+// nodes of one depth from their children (octree.h:104-126).
+__global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt,
+                                                   RefNode* __restrict__ nodes) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= cnt) return;
+  const int32_t id = level_list[beg + k];
+  const int32_t end = nodes[id].skip;
+  const double size = nodes[id].size;
+  double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+  // At most 8 children; the bound also keeps a malformed skip chain finite.
+  int32_t c = id + 1;
+  for (int it = 0; it < 8 && c < end; ++it, c = nodes[c].skip) {
+    const double cm = nodes[c].m;
+    if (cm == 0.0) continue;
+    M = M + cm;
+    sx = sx + nodes[c].cx * cm;
+    sy = sy + nodes[c].cy * cm;
+    sz = sz + nodes[c].cz * cm;
+  }
+  double cx = sx, cy = sy, cz = sz;
+  if (M > 0.0) { cx = sx / M; cy = sy / M; cz = sz / M; }
+  double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
+  const double soft = size * size * 0.01;
+  c = id + 1;
+  for (int it = 0; it < 8 && c < end; ++it, c = nodes[c].skip) {
+    const double mc = nodes[c].m;
+    if (mc == 0.0) continue;
+    const double rx = nodes[c].cx - cx, ry = nodes[c].cy - cy, rz = nodes[c].cz - cz;
+    const double r2 = rx * rx + ry * ry + rz * rz + soft;
+    qxx = qxx + mc * (3.0 * rx * rx - r2);
+    qyy = qyy + mc * (3.0 * ry * ry - r2);
+    qzz = qzz + mc * (3.0 * rz * rz - r2);
+    qxy = qxy + mc * (3.0 * rx * ry);
+    qxz = qxz + mc * (3.0 * rx * rz);
+    qyz = qyz + mc * (3.0 * ry * rz);
+  }
+  RefNode& p = nodes[id];
+  p.m = M; p.cx = cx; p.cy = cy; p.cz = cz;
+  p.esum = size * 0.015 + cbrt(M) * 0.002;
+  p.qxx = qxx; p.qyy = qyy; p.qzz = qzz; p.qxy = qxy; p.qxz = qxz; p.qyz = qyz;
+}
+
+// Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks
+// one contiguous (spatially coherent) range of targets and its L2 holds
+// the part of the tree those targets touch.
+__device__ __forceinline__ int64_t xcd_block(int nb) {
+  const int per = (nb + 7) / 8;
+  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+template <bool KICK, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const RefNode* __restrict__ nodes,
+                                                 StateArrays st, double theta, double* __restrict__ ax,
+                                                 double* __restrict__ ay, double* __restrict__ az, DevScalars* sc) {
+  const int64_t i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
+  const bool valid = i < n;
+  double px = 0, py = 0, pz = 0, pm = 1;
+  int pt = 0;
+  if (valid) { px = st.x[i]; py = st.y[i]; pz = st.z[i]; pm = st.m[i]; pt = st.type[i]; }
+  const int32_t me = valid ? (int32_t)i : -2;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  uint32_t npp = 0, npc = 0;
+  int32_t next = valid ? 0 : INT_MAX;
+  int32_t cur = __builtin_amdgcn_readfirstlane(wave_min_i32(next));
+  while (cur < total) {
+    const RefNode* nd = nodes + cur;
+    if (next == cur) {
+      const double nm = nd->m;
+      const int32_t body = nd->body;
+      // skip > cur by construction; the clamp keeps every lane moving forward
+      const int32_t skip = nd->skip > cur ? nd->skip : cur + 1;
+      if (nm == 0.0 || body == me) {                      // octree.h:134-135
+        next = skip;
+      } else {
+        const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double dist = sqrt(r2 + 1e-20);
+        const double size = nd->size;
+        const bool leaf = body >= 0;
+        if (leaf || (size / dist) < theta) {                // octree.h:153
+          double eps = nd->esum * (1.0 / (1.0 + dist * 10.0));
+          if (eps < 1e-4) eps = 1e-4;
+          if (pt == 1) {                                    // octree.h:146-148
+            const double alt = 2.0 * size / pow(nm / pm, 0.333333333);
+            if (eps < alt) eps = alt;
+          }
+          const double r2s = r2 + eps * eps;
+          const double dinv = 1.0 / sqrt(r2s);
+          const double inv3 = dinv * dinv * dinv;
+          const double fac = nm * inv3;                     // G = 1: (1.0 * m) * inv3
+          a0 = a0 + dx * fac;
+          a1 = a1 + dy * fac;
+          a2 = a2 + dz * fac;
+          if (!leaf) {                                      // leaves carry Q = 0
+            const double inv5 = inv3 * (dinv * dinv);
+            const double inv7 = inv5 * (dinv * dinv);
+            const double qxx = nd->qxx, qyy = nd->qyy, qzz = nd->qzz;
+            const double qxy = nd->qxy, qxz = nd->qxz, qyz = nd->qyz;
+            const double q = qxx * dx * dx + qyy * dy * dy + qzz * dz * dz +
+                             2.0 * (qxy * dx * dy + qxz * dx * dz + qyz * dy * dz);
+            const double qrx = 2.0 * (qxx * dx + qxy * dy + qxz * dz);
+            const double qry = 2.0 * (qxy * dx + qyy * dy + qyz * dz);
+            const double qrz = 2.0 * (qxz * dx + qyz * dy + qzz * dz);
+            a0 = a0 + 0.5 * (qrx * inv5 - 5.0 * q * inv7 * dx);
+            a1 = a1 + 0.5 * (qry * inv5 - 5.0 * q * inv7 * dy);
+            a2 = a2 + 0.5 * (qrz * inv5 - 5.0 * q * inv7 * dz);
+            if (COUNT) ++npc;
+          } else {
+            if (COUNT) ++npp;
+          }
+          next = skip;
+        } else {
+          next = cur + 1;                                   // open: first child
+        }
+      }
+    }
+    cur = __builtin_amdgcn_readfirstlane(wave_min_i32(next));
+  }
+  double v2 = 0.0;
+  if (valid) {
+    ax[i] = a0; ay[i] = a1; az[i] = a2;
+    if (KICK) {                                             // step.h:350-352
+      const double half = sc->dt * 0.5;
+      const double vx = st.vx[i] + a0 * half;
+      const double vy = st.vy[i] + a1 * half;
+      const double vz = st.vz[i] + a2 * half;
+      st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
+      v2 = vx * vx + vy * vy + vz * vz;
+      if (!(v2 > 0.0)) v2 = 0.0;
+    }
+  }
+  if (KICK) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double o = __shfl_xor(v2, off, kWave);
+      if (o > v2) v2 = o;
+    }
+    if (lane_id() == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
+  }
+  if (COUNT) {
+    const uint32_t sp = wave_sum_u32(npp), sc2 = wave_sum_u32(npc);
+    if (lane_id() == 0) {
+      atomicAdd(&sc->n_pp, (unsigned long long)sp);
+      atomicAdd(&sc->n_pc, (unsigned long long)sc2);
+    }
+  }
+}
+
+// Opening half kick and drift (step.h:294-325), with the partial bounding
+// box of the drifted positions for the next tree build.
+__global__ __launch_bounds__(kBlock) void k_kick_drift(int64_t n, StateArrays st, const double* __restrict__ ax,
+                                                       const double* __restrict__ ay, const double* __restrict__ az,
+                                                       const DevScalars* __restrict__ sc, double* part) {
+  const double dt = sc->dt;
+  const double half = dt * 0.5;
+  double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double vx = st.vx[i] + ax[i] * half;
+    const double vy = st.vy[i] + ay[i] * half;
+    const double vz = st.vz[i] + az[i] * half;
+    const double x = st.x[i] + vx * dt;
+    const double y = st.y[i] + vy * dt;
+    const double z = st.z[i] + vz * dt;
+    st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
+    st.x[i] = x; st.y[i] = y; st.z[i] = z;
+    lo_upd(lo[0], x); lo_upd(lo[1], y); lo_upd(lo[2], z);
+    hi_upd(hi[0], x); hi_upd(hi[1], y); hi_upd(hi[2], z);
+  }
+  block_bbox_write(lo, hi, part);
+}
+
+__global__ __launch_bounds__(kBlock) void k_maxv2(int64_t n, StateArrays st, DevScalars* sc) {
+  double v2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double s = st.vx[i] * st.vx[i] + st.vy[i] * st.vy[i] + st.vz[i] * st.vz[i];
+    if (s > v2) v2 = s;                                     // adaptive.h:34-37
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double o = __shfl_xor(v2, off, kWave);
+    if (o > v2) v2 = o;
+  }
+  if (lane_id() == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
+}
+
+// computeAdaptiveDt (adaptive.h:38-51) from the maximum |v|^2.
+__global__ void k_dt(DevScalars* sc, double base, double* dts, int64_t step) {
+  const double v2 = __longlong_as_double((long long)sc->v2max);
+  double v = sqrt(v2);
+  if (1e4 < v) v = 1e4;
+  double dt;
+  if (v < 1e-8) {
+    dt = base;
+  } else {
+    dt = base / (1.0 + v);
+    if (dt < base * 0.01) dt = base * 0.01;
+    if (base * 1.0 < dt) dt = base * 1.0;
+  }
+  sc->dt = dt;
+  sc->base_dt = base;
+  if (dts) dts[step] = dt;
+}
+
+__device__ __forceinline__ uint32_t digit_at(const uint64_t* key, const uint64_t* key2, int32_t f, int l) {
+  if (l < kKeyDigits) return (uint32_t)(key[f] >> (3 * (kKeyDigits - 1 - l))) & 7u;
+  return key2 ? (uint32_t)(key2[f] >> (3 * (kMaxLevels - 1 - l))) & 7u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_export(int64_t total, const RefNode* __restrict__ nodes,
+                                                   const uint64_t* __restrict__ key, const uint64_t* __restrict__ key2,
+                                                   const int32_t* __restrict__ id, const DevScalars* __restrict__ sc,
+                                                   int32_t* depth, int32_t* leaf, int32_t* body, int32_t* cmask,
+                                                   double* geo4, double* mass, double* com3, double* q6) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= total) return;
+  const RefNode nd = nodes[k];
+  depth[k] = nd.depth;
+  leaf[k] = nd.body >= 0 ? 1 : 0;
+  body[k] = nd.body >= 0 ? id[nd.body] : -1;
+  double cx = sc->root[0], cy = sc->root[1], cz = sc->root[2], s = sc->root[3];
+  for (int l = 0; l < nd.depth; ++l) {
+    const uint32_t o = digit_at(key, key2, nd.first, l);
+    const double hs = s * 0.5;
+    cx = cx + ((o & 1u) ? hs : -hs);
+    cy = cy + ((o & 2u) ? hs : -hs);
+    cz = cz + ((o & 4u) ? hs : -hs);
+    s = hs;
+  }
+  geo4[4 * k] = cx; geo4[4 * k + 1] = cy; geo4[4 * k + 2] = cz; geo4[4 * k + 3] = s;
+  int32_t mask = 0;
+  if (nd.body < 0)
+    for (int32_t c = (int32_t)k + 1, it = 0; it < 8 && c < nd.skip; ++it, c = nodes[c].skip)
+      mask |= 1 << digit_at(key, key2, nodes[c].first, nd.depth);
+  cmask[k] = mask;
+  mass[k] = nd.m;
+  com3[3 * k] = nd.cx; com3[3 * k + 1] = nd.cy; com3[3 * k + 2] = nd.cz;
+  q6[6 * k] = nd.qxx; q6[6 * k + 1] = nd.qyy; q6[6 * k + 2] = nd.qzz;
+  q6[6 * k + 3] = nd.qxy; q6[6 * k + 4] = nd.qxz; q6[6 * k + 5] = nd.qyz;
+}
+
+inline unsigned grid_of(int64_t n) { return (unsigned)div_up(n, kBlock); }
+
+}  // namespace
+
+void launch_fill_scalars(DevScalars* sc, int what, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(kBlock), 0, s, sc, what);
+}
+void launch_set_dt(DevScalars* sc, double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, sc, dt);
+}
+void launch_bbox(int64_t n, const StateArrays& st, double* part, DevScalars* sc, hipStream_t s) {
+  const int nb = (int)std::min<int64_t>(std::max<int64_t>(div_up(n, kBlock), 1), kBboxBlocks);
+  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kBlock), 0, s, n, st.x, st.y, st.z, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, part, nb, sc);
+}
+void launch_bbox_final(const double* part, int nparts, DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, sc);
+}
+void launch_keys(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* key, uint32_t* idx,
+                 DevScalars* scw, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sc, key, idx, scw);
+}
+void launch_keys2(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* key2, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_keys2, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sc, key2);
+}
+void launch_keys_export(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* hi, uint64_t* lo,
+                        hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_keys_export, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sc, hi, lo);
+}
+void launch_iota(int64_t n, uint32_t* idx, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(kBlock), 0, s, n, idx);
+}
+void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uint64_t* dst, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_u64, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
+}
+void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
+                         hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_state, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
+}
+void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, int32_t* lcp, uint32_t* cnt_s,
+                      uint32_t* cnt_e, DevScalars* sc, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_lcp_count, dim3(grid_of(n)), dim3(kBlock), 0, s, n, key, key2, lcp, cnt_s, cnt_e, sc);
+}
+void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
+                 const StateArrays& st, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
+                 uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sc, nodes,
+                       st_key, st_val, en_key, en_val);
+}
+void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
+                 hipStream_t s) {
+  if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, st_val, en_val, nodes, level_list);
+}
+void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, hipStream_t s) {
+  if (cnt > 0) hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes);
+}
+void launch_walk(int64_t n, int64_t total, const RefNode* nodes, const StateArrays& st, double theta, double* ax,
+                 double* ay, double* az, DevScalars* sc, bool kick, bool count, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned nb = (unsigned)(div_up(grid_of(n), 8) * 8);
+  const int32_t t = (int32_t)total;
+  if (kick && count)
+    hipLaunchKernelGGL((k_walk<true, true>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
+  else if (kick)
+    hipLaunchKernelGGL((k_walk<true, false>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
+  else if (count)
+    hipLaunchKernelGGL((k_walk<false, true>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
+  else
+    hipLaunchKernelGGL((k_walk<false, false>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
+}
+void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
+                       const DevScalars* sc, double* bbox_part, hipStream_t s) {
+  const int nb = (int)std::min<int64_t>(std::max<int64_t>(div_up(n, kBlock), 1), kBboxBlocks);
+  hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, s, n, st, ax, ay, az, sc, bbox_part);
+}
+void launch_maxv2(int64_t n, const StateArrays& st, DevScalars* sc, hipStream_t s) {
+  const int nb = (int)std::min<int64_t>(std::max<int64_t>(div_up(n, kBlock), 1), kBboxBlocks);
+  hipLaunchKernelGGL(k_maxv2, dim3(nb), dim3(kBlock), 0, s, n, st, sc);
+}
+void launch_dt(DevScalars* sc, double base, double* dts, int64_t step, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, s, sc, base, dts, step);
+}
+void launch_export(int64_t total, const RefNode* nodes, const uint64_t* key, const uint64_t* key2,
+                   const int32_t* id, const DevScalars* sc, int32_t* depth, int32_t* leaf, int32_t* body,
+                   int32_t* cmask, double* geo4, double* mass, double* com3, double* q6, hipStream_t s) {
+  if (total > 0)
+    hipLaunchKernelGGL(k_export, dim3(grid_of(total)), dim3(kBlock), 0, s, total, nodes, key, key2, id, sc, depth,
+                       leaf, body, cmask, geo4, mass, com3, q6);
+}
+
+}  // namespace nbx

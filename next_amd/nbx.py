@@ -1,0 +1,195 @@
+"""Python binding of the C ABI in include/nbx.h (ctypes over next_amd/libnbx.so).
+
+The calls mirror NEXT's path (begrun.cpp:96-99): ``adaptive_dt`` is
+computeAdaptiveDt (adaptive.h:23), ``step`` is Step (step.h:25); ``run``
+does both ``nsteps`` times on the device.  The library must be built
+(``python -m next_amd.build``); there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .ic import Particles
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnbx.so")
+
+STATUS = {0: "OK", -1: "NBX_ERR_ARG", -2: "NBX_ERR_NONFINITE", -3: "NBX_ERR_COINCIDENT",
+          -4: "NBX_ERR_NOMEM", -5: "NBX_ERR_HIP", -6: "NBX_ERR_NODEVICE", -7: "NBX_ERR_STATE"}
+
+MODE_REFERENCE, MODE_PHYSICS = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_upload",
+           "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
+           "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree"]
+
+
+class NbxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int), ("mode", C.c_int), ("precision", C.c_int), ("theta", C.c_double),
+                ("eps", C.c_double), ("leaf_size", C.c_int), ("group_size", C.c_int)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("keys_ms", C.c_double), ("sort_ms", C.c_double), ("build_ms", C.c_double),
+                ("upward_ms", C.c_double), ("walk_ms", C.c_double), ("kick_ms", C.c_double),
+                ("total_ms", C.c_double), ("nodes", C.c_int64), ("depth", C.c_int32), ("n_pp", C.c_int64),
+                ("n_pc", C.c_int64)]
+
+
+_lib = None
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+VP = C.c_void_p
+
+
+def lib():
+    """Load libnbx.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m next_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.nbx_config_default.argtypes = [C.POINTER(Config)]
+    L.nbx_config_default.restype = None
+    L.nbx_create.argtypes = [C.POINTER(VP), C.POINTER(Config)]
+    L.nbx_destroy.argtypes = [VP]
+    L.nbx_destroy.restype = None
+    L.nbx_last_error.argtypes = [VP]
+    L.nbx_last_error.restype = C.c_char_p
+    L.nbx_set_theta.argtypes = [VP, C.c_double]
+    L.nbx_upload.argtypes = [VP, C.c_int64] + [f64p] * 7 + [i32p]
+    L.nbx_download.argtypes = [VP] + [f64p] * 6
+    L.nbx_size.argtypes = [VP]
+    L.nbx_size.restype = C.c_int64
+    L.nbx_adaptive_dt.argtypes = [VP, C.c_double, C.POINTER(C.c_double)]
+    L.nbx_step.argtypes = [VP, C.c_double]
+    L.nbx_run.argtypes = [VP, C.c_double, C.c_int32, f64p]
+    L.nbx_accel.argtypes = [VP, f64p, f64p, f64p]
+    L.nbx_get_timing.argtypes = [VP, C.POINTER(Timing)]
+    L.nbx_reset_timing.argtypes = [VP]
+    L.nbx_set_profiling.argtypes = [VP, C.c_int]
+    L.nbx_debug_keys.argtypes = [VP, f64p, u64p, u64p]
+    L.nbx_debug_tree_size.argtypes = [VP, C.POINTER(C.c_int64)]
+    L.nbx_debug_tree.argtypes = [VP, i32p, i32p, i32p, i32p, f64p, f64p, f64p, f64p, i32p]
+    _lib = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Engine:
+    """One device-resident particle system (an nbx_ctx)."""
+
+    def __init__(self, theta: float = 0.5, device: int = 0, mode: int = MODE_REFERENCE,
+                 precision: int = PREC_FP64, eps: float = 0.01, leaf_size: int = 16, group_size: int = 64):
+        L = lib()
+        cfg = Config()
+        L.nbx_config_default(C.byref(cfg))
+        cfg.device, cfg.mode, cfg.precision = device, mode, precision
+        cfg.theta, cfg.eps, cfg.leaf_size, cfg.group_size = theta, eps, leaf_size, group_size
+        h = VP()
+        st = L.nbx_create(C.byref(h), C.byref(cfg))
+        if st != 0:
+            raise NbxError(st, "nbx_create failed")
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nbx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            raise NbxError(st, lib().nbx_last_error(self._h).decode())
+
+    def set_theta(self, theta: float):
+        self._check(lib().nbx_set_theta(self._h, theta))
+
+    def upload(self, p: Particles):
+        n = len(p)
+        arrs = [_f64(a) for a in (p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m)]
+        t = np.ascontiguousarray(p.type, dtype=np.int32)
+        if n == 0:
+            arrs = [np.zeros(1) for _ in range(7)]
+            t = np.zeros(1, np.int32)
+        self._check(lib().nbx_upload(self._h, n, *arrs, t))
+        self.n = n
+
+    def download(self):
+        out = [np.zeros(max(self.n, 1)) for _ in range(6)]
+        self._check(lib().nbx_download(self._h, *out))
+        return np.stack([o[: self.n] for o in out])
+
+    def adaptive_dt(self, base_dt: float) -> float:
+        dt = C.c_double()
+        self._check(lib().nbx_adaptive_dt(self._h, base_dt, C.byref(dt)))
+        return dt.value
+
+    def step(self, dt: float):
+        self._check(lib().nbx_step(self._h, dt))
+
+    def run(self, base_dt: float, nsteps: int):
+        dts = np.zeros(max(nsteps, 1))
+        self._check(lib().nbx_run(self._h, base_dt, nsteps, dts))
+        return dts[:nsteps]
+
+    def accel(self):
+        out = [np.zeros(max(self.n, 1)) for _ in range(3)]
+        self._check(lib().nbx_accel(self._h, *out))
+        return np.stack([o[: self.n] for o in out], axis=1)
+
+    def set_profiling(self, flags: int):
+        self._check(lib().nbx_set_profiling(self._h, flags))
+
+    def reset_timing(self):
+        self._check(lib().nbx_reset_timing(self._h))
+
+    def timing(self) -> dict:
+        t = Timing()
+        self._check(lib().nbx_get_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def debug_keys(self):
+        root = np.zeros(4)
+        n = max(self.n, 1)
+        hi = np.zeros(n, np.uint64)
+        lo = np.zeros(n, np.uint64)
+        self._check(lib().nbx_debug_keys(self._h, root, hi, lo))
+        return root, hi[: self.n], lo[: self.n]
+
+    def debug_tree(self) -> dict:
+        k = C.c_int64()
+        self._check(lib().nbx_debug_tree_size(self._h, C.byref(k)))
+        k = k.value
+        kk = max(k, 1)
+        t = dict(depth=np.zeros(kk, np.int32), leaf=np.zeros(kk, np.int32), body=np.zeros(kk, np.int32),
+                 cmask=np.zeros(kk, np.int32), geo=np.zeros((kk, 4)), mass=np.zeros(kk), com=np.zeros((kk, 3)),
+                 q=np.zeros((kk, 6)), perm=np.zeros(max(self.n, 1), np.int32))
+        self._check(lib().nbx_debug_tree(self._h, t["depth"], t["leaf"], t["body"], t["cmask"], t["geo"],
+                                         t["mass"], t["com"], t["q"], t["perm"]))
+        out = {key: v[:k] for key, v in t.items() if key != "perm"}
+        out["perm"] = t["perm"][: self.n]
+        return out

@@ -314,6 +314,19 @@ void orc_tree_accel(const orc_tree* t, int64_t n, const double* x, const double*
   accel_all(t->t, n, x, y, z, m, type, theta, ax, ay, az, threads);
 }
 
+// The walk for a subset of targets (indices into the particle arrays);
+// results go to ax[k], ay[k], az[k] for targets[k].
+void orc_tree_accel_subset(const orc_tree* t, const double* x, const double* y, const double* z,
+                           const double* m, const int32_t* type, double theta, int64_t nt,
+                           const int64_t* targets, double* ax, double* ay, double* az, int threads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+  for (int64_t k = 0; k < nt; ++k) {
+    double a0 = 0, a1 = 0, a2 = 0;
+    walk(t->t, 0, (int)targets[k], x, y, z, m, type, theta, a0, a1, a2);
+    ax[k] = a0; ay[k] = a1; az[k] = a2;
+  }
+}
+
 void orc_tree_free(orc_tree* t) { delete t; }
 
 // Global time step from the largest speed (adaptive.h:23-52).

@@ -38,6 +38,9 @@ def lib():
         L.orc_tree_accel.argtypes = [C.c_void_p, C.c_int64, f64p, f64p, f64p, f64p, i32p, C.c_double,
                                      f64p, f64p, f64p, C.c_int]
         L.orc_tree_free.argtypes = [C.c_void_p]
+        L.orc_tree_accel_subset.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, i32p, C.c_double, C.c_int64,
+                                            np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"), f64p, f64p,
+                                            f64p, C.c_int]
         L.orc_adaptive_dt.argtypes = [C.c_int64, f64p, f64p, f64p, C.c_double]
         L.orc_adaptive_dt.restype = C.c_double
         L.orc_step.argtypes = [C.c_int64] + [f64p] * 7 + [i32p, C.c_double, C.c_double, C.c_int]
@@ -100,6 +103,23 @@ def accel(x, y, z, m, typ, theta, threads=8):
             raise RuntimeError("oracle: coincident bodies (depth cap)")
         if n > 0:
             L.orc_tree_accel(h, n, x, y, z, m, typ, theta, *out, threads)
+    finally:
+        L.orc_tree_free(h)
+    return np.stack(out, axis=1)
+
+
+def accel_subset(x, y, z, m, typ, theta, targets, threads=8):
+    """Reference-order walk for the given target indices only."""
+    L = lib()
+    x, y, z, m = (_c(a, np.float64) for a in (x, y, z, m))
+    typ = _c(typ, np.int32)
+    tg = _c(targets, np.int64)
+    out = [np.zeros(len(tg)) for _ in range(3)]
+    h = L.orc_tree_build(len(x), x, y, z, m)
+    try:
+        if L.orc_tree_error(h):
+            raise RuntimeError("oracle: coincident bodies (depth cap)")
+        L.orc_tree_accel_subset(h, x, y, z, m, typ, theta, len(tg), tg, *out, threads)
     finally:
         L.orc_tree_free(h)
     return np.stack(out, axis=1)
