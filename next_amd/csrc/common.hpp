@@ -14,8 +14,10 @@ constexpr int kMaxDepthSlots = 64; // histogram slots for node depths
 
 // One tree node in depth-first preorder, laid out for the walk: a wave reads
 // one node at a time at a wave-uniform index, so the record is fetched
-// through the scalar cache and broadcast to all lanes.
-struct alignas(16) RefNode {
+// through the scalar cache and broadcast to all lanes.  The fields every
+// visit needs sit in the first 64 bytes; the quadrupole, needed only when a
+// cell is accepted, in the second.  128 bytes = one L2 line.
+struct alignas(128) RefNode {
   double cx, cy, cz, m;    // centre of mass, mass
   double size;             // half-width: root half-width * 2^-depth
   double esum;             // size*0.015 + cbrt(m)*0.002 (softening numerator)
@@ -24,8 +26,9 @@ struct alignas(16) RefNode {
   int32_t depth;
   int32_t first;           // first body (sorted index) of the subtree
   double qxx, qyy, qzz, qxy, qxz, qyz;
+  double pad[2];
 };
-static_assert(sizeof(RefNode) == 112, "RefNode layout");
+static_assert(sizeof(RefNode) == 128, "RefNode layout");
 
 // Small per-context device block: geometry, time step, flags and the
 // per-level node counts the host needs to launch the upward pass.
@@ -42,6 +45,7 @@ struct DevScalars {
   int32_t max_leaf_depth;
   int32_t level_count[kMaxDepthSlots];  // internal nodes per depth
   unsigned long long n_pp, n_pc;        // interaction counters
+  unsigned long long n_open, n_iter;    // opened nodes, wave iterations
 };
 
 // ---- wave primitives ------------------------------------------------------

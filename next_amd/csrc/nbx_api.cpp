@@ -33,7 +33,7 @@ struct DBuf {
   size_t bytes = 0;
 };
 
-enum { EV_START, EV_KICK, EV_KEYS, EV_SORT, EV_BUILD, EV_UPWARD, EV_WALK, EV_COUNT };
+enum { EV_START, EV_KICK, EV_KEYS, EV_SORT, EV_BUILD, EV_UPWARD, EV_WALK0, EV_WALK, EV_COUNT };
 
 }  // namespace
 
@@ -50,7 +50,7 @@ struct nbx_ctx {
   DBuf lcp, cnt_s, cnt_e;
   DBuf nodes;
   DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
-  DBuf scratch, bbox_part, dts;
+  DBuf scratch, bbox_part, dts, lvl_part;
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
   bool acc_valid = false;     // acc holds a(x) of the current positions
@@ -138,6 +138,7 @@ nbx_status ensure_particles(nbx_ctx* c, int64_t n) {
   NBXCHK(ensure(c, c->cnt_s, nn * 4));
   NBXCHK(ensure(c, c->cnt_e, nn * 4));
   NBXCHK(ensure(c, c->bbox_part, (size_t)kBboxBlocks * 6 * 8));
+  NBXCHK(ensure(c, c->lvl_part, (size_t)lcp_part_words(std::max<int64_t>(n, 1)) * 4));
   NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(n), scan_scratch_words(n)) * 4));
   return NBX_OK;
 }
@@ -164,7 +165,8 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
 
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
   uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
-  launch_lcp_count(n, skey, nullptr, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), c->sc, s);
+  launch_lcp_count(n, skey, nullptr, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
+                   P<uint32_t>(c->lvl_part), c->sc, s);
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
   scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
   NBXCHK(sync_scalars(c));
@@ -195,7 +197,8 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
     launch_keys2(n, S, c->sc, P<uint64_t>(c->key2[0]), s);
     skey2 = P<uint64_t>(c->key2[0]);
     launch_fill_scalars(c->sc, 1, s);
-    launch_lcp_count(n, skey, skey2, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), c->sc, s);
+    launch_lcp_count(n, skey, skey2, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
+                     P<uint32_t>(c->lvl_part), c->sc, s);
     scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
     scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
     NBXCHK(sync_scalars(c));
@@ -241,6 +244,7 @@ nbx_status walk(nbx_ctx* c, bool kick) {
   const bool count = (c->prof & 2) != 0;
   if (count) launch_fill_scalars(c->sc, 8, c->stream);
   if (kick) launch_fill_scalars(c->sc, 2, c->stream);
+  record(c, EV_WALK0);
   launch_walk(c->n, c->total_nodes, P<RefNode>(c->nodes), view(c, c->cur), c->cfg.theta, P<double>(c->acc[0]),
               P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
   record(c, EV_WALK);
@@ -264,7 +268,7 @@ nbx_status finish_timing(nbx_ctx* c, bool with_kick) {
     c->tim.sort_ms += elapsed(c, EV_KEYS, EV_SORT);
     c->tim.build_ms += elapsed(c, EV_SORT, EV_BUILD);
     c->tim.upward_ms += elapsed(c, EV_BUILD, EV_UPWARD);
-    c->tim.walk_ms += elapsed(c, EV_UPWARD, EV_WALK);
+    c->tim.walk_ms += elapsed(c, EV_WALK0, EV_WALK);
     c->tim.total_ms += elapsed(c, EV_START, EV_WALK);
   }
   if (c->prof & 2) {
@@ -272,6 +276,8 @@ nbx_status finish_timing(nbx_ctx* c, bool with_kick) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tim.n_pp += (int64_t)c->hsc->n_pp;
     c->tim.n_pc += (int64_t)c->hsc->n_pc;
+    c->tim.n_open += (int64_t)c->hsc->n_open;
+    c->tim.n_iter += (int64_t)c->hsc->n_iter;
   }
   c->tim.steps += 1;
   c->tim.nodes = c->total_nodes;
@@ -377,7 +383,7 @@ void nbx_destroy(nbx_ctx* c) {
   for (auto& b : c->key2) release(b);
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->bbox_part,
-                  &c->dts})
+                  &c->dts, &c->lvl_part})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->sc) (void)hipFree(c->sc);

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int wha
   }
   if ((what & 2) && t == 0) sc->v2max = 0ull;
   if ((what & 4) && t == 0) sc->err = 0;
-  if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; }
+  if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; }
 }
 
 __global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
@@ -201,16 +201,15 @@ __device__ __forceinline__ int lcp_pair(const uint64_t* key, const uint64_t* key
 
 // lcp[i] = digits shared by sorted bodies i and i+1 (-1 past the end).
 // Body i starts the internal nodes at depths (lcp[i-1], lcp[i]] and its leaf;
-// it ends the internal nodes at depths (lcp[i], lcp[i-1]].
+// it ends the internal nodes at depths (lcp[i], lcp[i-1]].  Each block also
+// writes one row of partial counts (internal nodes per depth, deepest leaf,
+// equal-key pairs) for k_lcp_reduce: plain stores, no global atomics.
 __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t* __restrict__ key,
                                                       const uint64_t* __restrict__ key2, int32_t* __restrict__ lcp,
                                                       uint32_t* __restrict__ cnt_s, uint32_t* __restrict__ cnt_e,
-                                                      DevScalars* sc) {
-  __shared__ uint32_t hist[kMaxDepthSlots];
-  __shared__ int32_t smax;
-  __shared__ uint32_t scoll;
-  if (threadIdx.x < kMaxDepthSlots) hist[threadIdx.x] = 0;
-  if (threadIdx.x == 0) { smax = 0; scoll = 0; }
+                                                      uint32_t* __restrict__ part) {
+  __shared__ uint32_t hist[kPartW];
+  if (threadIdx.x < kPartW) hist[threadIdx.x] = 0;
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) {
@@ -221,14 +220,37 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t*
     cnt_s[i] = 1u + (uint32_t)(li > lp ? li - lp : 0);
     cnt_e[i] = (uint32_t)(lp > li ? lp - li : 0);
     for (int d = lp + 1; d <= li; ++d) atomicAdd(&hist[d], 1u);
-    atomicMax(&smax, 1 + (li > lp ? li : lp));
-    if (eq_i) atomicAdd(&scoll, 1u);
+    atomicMax(&hist[kPartMaxDepth], (uint32_t)(1 + (li > lp ? li : lp)));
+    if (eq_i) atomicAdd(&hist[kPartEqual], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < kMaxDepthSlots && hist[threadIdx.x]) atomicAdd(&sc->level_count[threadIdx.x], (int)hist[threadIdx.x]);
+  if (threadIdx.x < kPartW) part[(int64_t)blockIdx.x * kPartW + threadIdx.x] = hist[threadIdx.x];
+}
+
+// Column b of the partial rows: one block per column.
+__global__ __launch_bounds__(kBlock) void k_lcp_reduce(const uint32_t* __restrict__ part, int64_t rows, int has_key2,
+                                                       DevScalars* sc) {
+  __shared__ uint32_t red[kBlock / kWave];
+  const int col = blockIdx.x;
+  const bool is_max = col == kPartMaxDepth;
+  uint32_t v = 0;
+  for (int64_t r = threadIdx.x; r < rows; r += kBlock) {
+    const uint32_t x = part[r * kPartW + col];
+    v = is_max ? (x > v ? x : v) : v + x;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off, kWave);
+    v = is_max ? (o > v ? o : v) : v + o;
+  }
+  if (lane_id() == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    atomicMax(&sc->max_leaf_depth, smax);
-    if (scoll) atomicAdd(key2 ? &sc->coincident : &sc->collisions, (int)scoll);
+    for (int w = 1; w < kBlock / kWave; ++w) v = is_max ? (red[w] > v ? red[w] : v) : v + red[w];
+    if (col < kMaxDepthSlots) sc->level_count[col] = (int32_t)v;
+    else if (is_max) sc->max_leaf_depth = (int32_t)v;
+    else if (has_key2) sc->coincident = (int32_t)v;
+    else sc->collisions = (int32_t)v;
   }
 }
 
@@ -251,32 +273,30 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
     const uint32_t p = pre + t;
     st_key[sbase + t] = (uint32_t)d;
     st_val[sbase + t] = p;
-    RefNode nd;
-    nd.cx = nd.cy = nd.cz = nd.m = 0.0;
+    // mass, centre of mass, softening numerator and quadrupole come from
+    // k_upward, the skip pointer from k_pair
+    RefNode& nd = nodes[p];
     nd.size = ldexp(s0, -d);
-    nd.esum = 0.0;
-    nd.skip = 0;
     nd.body = -1;
     nd.depth = d;
     nd.first = (int32_t)i;
-    nd.qxx = nd.qyy = nd.qzz = nd.qxy = nd.qxz = nd.qyz = 0.0;
-    nodes[p] = nd;
   }
   {
     const uint32_t p = pre + ns;
     const int d = 1 + (li > lp ? li : lp);
-    RefNode nd;
+    // Leaves carry no quadrupole; the walk never reads a leaf's Q fields.
+    RefNode& nd = nodes[p];
+    const double m = st.m[i];
+    const double size = ldexp(s0, -d);
     nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
-    nd.m = st.m[i];
-    nd.size = ldexp(s0, -d);
+    nd.m = m;
+    nd.size = size;
     // softening.h:26-27: size*0.015 + cbrt(m)*0.002, summed before the taper
-    nd.esum = nd.size * 0.015 + cbrt(nd.m) * 0.002;
+    nd.esum = size * 0.015 + cbrt(m) * 0.002;
     nd.skip = (int32_t)(p + 1);
     nd.body = (int32_t)i;
     nd.depth = d;
     nd.first = (int32_t)i;
-    nd.qxx = nd.qyy = nd.qzz = nd.qxy = nd.qxz = nd.qyz = 0.0;
-    nodes[p] = nd;
   }
   const int ne = lp > li ? lp - li : 0;
   if (ne) {
@@ -351,6 +371,33 @@ __device__ __forceinline__ int64_t xcd_block(int nb) {
   return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// The per-visit fields of a node, held in scalar registers.
+struct NodeHead {
+  double cx, cy, cz, m, size, esum;
+  int32_t skip, body;
+};
+
+__device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes, int32_t k) {
+  const RefNode* p = nodes + k;
+  NodeHead h;
+  h.cx = p->cx; h.cy = p->cy; h.cz = p->cz; h.m = p->m;
+  h.size = p->size; h.esum = p->esum; h.skip = p->skip; h.body = p->body;
+  return h;
+}
+
+// Union traversal.  Every lane follows its own depth-first path (open: go to
+// the first child, cur+1; accept or skip: go past the subtree, skip[cur]).
+// The wave visits node `cur` and the lanes whose own next node is `cur`
+// process it.  No lane that is waiting can point inside cur's subtree (its
+// pointer came from a skip of an ancestor or an earlier sibling, both of
+// which end at or after skip[cur]), so the wave's next node is cur+1 when
+// any lane opened cur and skip[cur] otherwise: one ballot, no reduction.
+//
+// Opening test of octree.h:153, (size/dist) < theta with
+// dist = sqrt(r2 + 1e-20): decided from r2 + 1e-20 against (size/theta)^2
+// when it lies outside a relative band of 1e-12 (rounding errors of either
+// form are below 1e-15), and by the exact expression inside the band, so the
+// decision is NEXT's bit for bit and opened nodes cost no sqrt or divide.
 template <bool KICK, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const RefNode* __restrict__ nodes,
                                                  StateArrays st, double theta, double* __restrict__ ax,
@@ -361,44 +408,55 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
   int pt = 0;
   if (valid) { px = st.x[i]; py = st.y[i]; pz = st.z[i]; pm = st.m[i]; pt = st.type[i]; }
   const int32_t me = valid ? (int32_t)i : -2;
+  const double inv_theta = 1.0 / theta;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  uint32_t npp = 0, npc = 0;
+  uint32_t npp = 0, npc = 0, nop = 0, nit = 0;
   int32_t next = valid ? 0 : INT_MAX;
-  int32_t cur = __builtin_amdgcn_readfirstlane(wave_min_i32(next));
+  int32_t cur = __ballot(valid) ? 0 : total;
+  NodeHead h = load_head(nodes, 0);
   while (cur < total) {
-    const RefNode* nd = nodes + cur;
+    // successors, loaded while this node is processed
+    const int32_t sk = h.skip > cur ? h.skip : cur + 1;   // skip > cur by construction
+    const NodeHead h1 = load_head(nodes, cur + 1 < total ? cur + 1 : cur);
+    const NodeHead h2 = load_head(nodes, sk < total ? sk : cur);
+    bool open = false;
     if (next == cur) {
-      const double nm = nd->m;
-      const int32_t body = nd->body;
-      // skip > cur by construction; the clamp keeps every lane moving forward
-      const int32_t skip = nd->skip > cur ? nd->skip : cur + 1;
-      if (nm == 0.0 || body == me) {                      // octree.h:134-135
-        next = skip;
+      if (h.m == 0.0 || h.body == me) {                    // octree.h:134-135
+        next = sk;
       } else {
-        const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
+        const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const double dist = sqrt(r2 + 1e-20);
-        const double size = nd->size;
-        const bool leaf = body >= 0;
-        if (leaf || (size / dist) < theta) {                // octree.h:153
-          double eps = nd->esum * (1.0 / (1.0 + dist * 10.0));
+        const double r2e = r2 + 1e-20;
+        const bool leaf = h.body >= 0;
+        bool accept = leaf;
+        if (!leaf) {
+          const double q = h.size * inv_theta;
+          const double q2 = q * q;
+          if (r2e > q2 * (1.0 + 1e-12)) accept = true;
+          else if (r2e < q2 * (1.0 - 1e-12)) accept = false;
+          else accept = (h.size / sqrt(r2e)) < theta;
+        }
+        if (accept) {
+          const double dist = sqrt(r2e);
+          double eps = h.esum * (1.0 / (1.0 + dist * 10.0));  // softening.h:26-36
           if (eps < 1e-4) eps = 1e-4;
           if (pt == 1) {                                    // octree.h:146-148
-            const double alt = 2.0 * size / pow(nm / pm, 0.333333333);
+            const double alt = 2.0 * h.size / pow(h.m / pm, 0.333333333);
             if (eps < alt) eps = alt;
           }
           const double r2s = r2 + eps * eps;
           const double dinv = 1.0 / sqrt(r2s);
           const double inv3 = dinv * dinv * dinv;
-          const double fac = nm * inv3;                     // G = 1: (1.0 * m) * inv3
+          const double fac = h.m * inv3;                    // G = 1: (1.0 * m) * inv3
           a0 = a0 + dx * fac;
           a1 = a1 + dy * fac;
           a2 = a2 + dz * fac;
           if (!leaf) {                                      // leaves carry Q = 0
-            const double inv5 = inv3 * (dinv * dinv);
-            const double inv7 = inv5 * (dinv * dinv);
+            const RefNode* nd = nodes + cur;
             const double qxx = nd->qxx, qyy = nd->qyy, qzz = nd->qzz;
             const double qxy = nd->qxy, qxz = nd->qxz, qyz = nd->qyz;
+            const double inv5 = inv3 * (dinv * dinv);
+            const double inv7 = inv5 * (dinv * dinv);
             const double q = qxx * dx * dx + qyy * dy * dy + qzz * dz * dz +
                              2.0 * (qxy * dx * dy + qxz * dx * dz + qyz * dy * dz);
             const double qrx = 2.0 * (qxx * dx + qxy * dy + qxz * dz);
@@ -411,13 +469,17 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
           } else {
             if (COUNT) ++npp;
           }
-          next = skip;
+          next = sk;
         } else {
-          next = cur + 1;                                   // open: first child
+          open = true;                                      // first child
+          next = cur + 1;
+          if (COUNT) ++nop;
         }
       }
     }
-    cur = __builtin_amdgcn_readfirstlane(wave_min_i32(next));
+    if (COUNT) ++nit;
+    if (__ballot(open)) { cur = cur + 1; h = h1; }
+    else { cur = sk; h = h2; }
   }
   double v2 = 0.0;
   if (valid) {
@@ -441,10 +503,12 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
     if (lane_id() == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
   }
   if (COUNT) {
-    const uint32_t sp = wave_sum_u32(npp), sc2 = wave_sum_u32(npc);
+    const uint32_t sp = wave_sum_u32(npp), sc2 = wave_sum_u32(npc), so = wave_sum_u32(nop);
     if (lane_id() == 0) {
       atomicAdd(&sc->n_pp, (unsigned long long)sp);
       atomicAdd(&sc->n_pc, (unsigned long long)sc2);
+      atomicAdd(&sc->n_open, (unsigned long long)so);
+      atomicAdd(&sc->n_iter, (unsigned long long)nit);
     }
   }
 }
@@ -537,8 +601,9 @@ __global__ __launch_bounds__(kBlock) void k_export(int64_t total, const RefNode*
   cmask[k] = mask;
   mass[k] = nd.m;
   com3[3 * k] = nd.cx; com3[3 * k + 1] = nd.cy; com3[3 * k + 2] = nd.cz;
-  q6[6 * k] = nd.qxx; q6[6 * k + 1] = nd.qyy; q6[6 * k + 2] = nd.qzz;
-  q6[6 * k + 3] = nd.qxy; q6[6 * k + 4] = nd.qxz; q6[6 * k + 5] = nd.qyz;
+  const bool lf = nd.body >= 0;
+  q6[6 * k] = lf ? 0.0 : nd.qxx; q6[6 * k + 1] = lf ? 0.0 : nd.qyy; q6[6 * k + 2] = lf ? 0.0 : nd.qzz;
+  q6[6 * k + 3] = lf ? 0.0 : nd.qxy; q6[6 * k + 4] = lf ? 0.0 : nd.qxz; q6[6 * k + 5] = lf ? 0.0 : nd.qyz;
 }
 
 inline unsigned grid_of(int64_t n) { return (unsigned)div_up(n, kBlock); }
@@ -581,9 +646,11 @@ void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src
   if (n > 0) hipLaunchKernelGGL(k_gather_state, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
 }
 void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, int32_t* lcp, uint32_t* cnt_s,
-                      uint32_t* cnt_e, DevScalars* sc, hipStream_t s) {
-  if (n > 0)
-    hipLaunchKernelGGL(k_lcp_count, dim3(grid_of(n)), dim3(kBlock), 0, s, n, key, key2, lcp, cnt_s, cnt_e, sc);
+                      uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t rows = grid_of(n);
+  hipLaunchKernelGGL(k_lcp_count, dim3((unsigned)rows), dim3(kBlock), 0, s, n, key, key2, lcp, cnt_s, cnt_e, part);
+  hipLaunchKernelGGL(k_lcp_reduce, dim3(kPartW), dim3(kBlock), 0, s, part, rows, key2 ? 1 : 0, sc);
 }
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
                  const StateArrays& st, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,

@@ -18,6 +18,11 @@ struct StateArrays {
 
 constexpr int kBlock = 256;
 constexpr int kBboxBlocks = 1024;   // partial min/max blocks (grid-stride)
+// Row of per-block partial counts written by the lcp kernel.
+constexpr int kPartMaxDepth = kMaxDepthSlots;      // column: deepest leaf
+constexpr int kPartEqual = kMaxDepthSlots + 1;     // column: equal-key pairs
+constexpr int kPartW = kMaxDepthSlots + 2;
+inline int64_t lcp_part_words(int64_t n) { return ((n + kBlock - 1) / kBlock) * kPartW; }
 
 void launch_fill_scalars(DevScalars* sc, int what, hipStream_t s);  // what: 1 build counters, 2 v2max, 4 err
 void launch_set_dt(DevScalars* sc, double dt, hipStream_t s);
@@ -30,7 +35,7 @@ void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uin
 void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
                          hipStream_t s);
 void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, int32_t* lcp, uint32_t* cnt_s,
-                      uint32_t* cnt_e, DevScalars* sc, hipStream_t s);
+                      uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s);
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
                  const StateArrays& st, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
                  uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, hipStream_t s);

@@ -37,9 +37,26 @@
 namespace {
 
 // Text IC reader with the stop-at-first-failure semantics of
-// load_particle.hpp:100-116 (the header itself pulls in HDF5).
+// load_particle.hpp:100-116 (the header itself pulls in HDF5).  A path
+// ending in ".bin" is read as raw arrays instead: int64 n, then x y z vx vy
+// vz m (n doubles each) and type (n int32).
 Particle load_text(const char* path) {
   Particle p;
+  const std::string ps(path);
+  if (ps.size() > 4 && ps.compare(ps.size() - 4, 4, ".bin") == 0) {
+    FILE* f = std::fopen(path, "rb");
+    int64_t n = 0;
+    if (!f || std::fread(&n, 8, 1, f) != 1) { if (f) std::fclose(f); return p; }
+    std::vector<double> cols(7 * (size_t)n);
+    std::vector<int32_t> t((size_t)n);
+    if (std::fread(cols.data(), 8, cols.size(), f) != cols.size() ||
+        std::fread(t.data(), 4, t.size(), f) != t.size()) { std::fclose(f); return p; }
+    std::fclose(f);
+    for (int64_t i = 0; i < n; ++i)
+      p.addParticle(cols[i], cols[n + i], cols[2 * n + i], cols[3 * n + i], cols[4 * n + i],
+                    cols[5 * n + i], cols[6 * n + i], t[i]);
+    return p;
+  }
   std::ifstream in(path);
   double a[7];
   int t;

@@ -1,0 +1,66 @@
+"""The C-ABI library loads and exports every entry point include/nbx.h
+declares (no device needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from next_amd import nbx
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "nbx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbx_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_binding_set():
+    assert set(declared_functions()) == set(nbx.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(nbx.LIB_PATH)
+    missing = [f for f in declared_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_config_defaults_are_nexts():
+    cfg = nbx.Config()
+    nbx.lib().nbx_config_default(ctypes.byref(cfg))
+    assert cfg.theta == 0.5          # step.h:211
+    assert cfg.mode == nbx.MODE_REFERENCE and cfg.precision == nbx.PREC_FP64 and cfg.device == 0
+
+
+def test_null_and_bad_arguments_are_rejected_without_a_device():
+    L = nbx.lib()
+    assert L.nbx_create(None, None) == -1
+    assert L.nbx_get_timing(None, None) == -1
+    assert L.nbx_set_theta(None, 0.5) == -1
+    assert L.nbx_size(None) == -1
+    assert L.nbx_last_error(None) == b"null context"
+    cfg = nbx.Config()
+    L.nbx_config_default(ctypes.byref(cfg))
+    cfg.theta = -1.0
+    h = ctypes.c_void_p()
+    assert L.nbx_create(ctypes.byref(h), ctypes.byref(cfg)) == -1
+
+
+def test_create_reports_the_device_state():
+    """On a machine without a GPU nbx_create fails with NBX_ERR_NODEVICE; on
+    a GPU box it succeeds."""
+    try:
+        e = nbx.Engine()
+    except nbx.NbxError as err:
+        assert err.status == -6
+    else:
+        e.close()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(nbx, "_lib", None)
+    monkeypatch.setattr(nbx, "LIB_PATH", str(tmp_path / "libnbx.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        nbx.lib()

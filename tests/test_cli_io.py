@@ -1,0 +1,95 @@
+"""The `next` driver's command line, text loader and snapshot writers
+(host code; no device needed).
+
+Snapshots are compared byte for byte with the files NEXT's own SaveVTU /
+SaveVTK wrote for the same particles (tests/golden/writers_small.*).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from next_amd import ic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NEXT = os.path.join(ROOT, "next_amd", "bin", "next")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run(*args, cwd=None):
+    return subprocess.run([NEXT, *map(str, args)], capture_output=True, text=True, cwd=cwd, timeout=60)
+
+
+def test_usage_on_wrong_argument_count():
+    r = run()
+    assert r.returncode == 1
+    assert "Usage: next <input.txt> <threads> <dt> <dump_interval> <vtk|vtu|hdf5>" in r.stderr
+    assert run("a", "1", "0.1").returncode == 1
+
+
+def test_bad_format_and_numbers():
+    r = run("x.txt", "1", "0.001", "0.1", "pdf")
+    assert r.returncode == 1 and "Choose a file format: vtk, vtu, or hdf5" in r.stderr
+    assert run("x.txt", "one", "0.001", "0.1", "vtu").returncode == 1
+    assert run("x.txt", "1", "0.001", "0.1", "vtu", "--steps").returncode == 1
+    assert run("x.txt", "1", "0.001", "0.1", "vtu", "--bogus").returncode == 1
+
+
+def test_hdf5_is_refused_clearly():
+    r = run("x.txt", "1", "0.001", "0.1", "hdf5")
+    assert r.returncode == 1 and "hdf5" in r.stderr
+
+
+@pytest.mark.parametrize("fmt", ["vtu", "vtk"])
+def test_writers_match_reference_bytes(tmp_path, fmt):
+    out = tmp_path / f"w.{fmt}"
+    r = run(os.path.join(GOLD, "writers_small.ic.txt"), 1, 0.001, 0.01, fmt, "--dump-ic", out)
+    assert r.returncode == 0, r.stderr
+    assert " Particles: 5" in r.stdout
+    assert out.read_bytes() == open(os.path.join(GOLD, f"writers_small.{fmt}"), "rb").read()
+
+
+def test_startup_lines(tmp_path):
+    r = run(os.path.join(GOLD, "writers_small.ic.txt"), 4, 0.001, 0.01, "vtu", "--dump-ic", tmp_path / "a.vtu")
+    lines = r.stdout.splitlines()
+    assert " Threads:   4" in lines and " Precision: FP64" in lines and " Particles: 5" in lines
+
+
+def _count(tmp_path, text):
+    p = tmp_path / "ic.txt"
+    p.write_text(text)
+    r = run(p, 1, 0.001, 0.1, "vtk", "--dump-ic", tmp_path / "o.vtk")
+    assert r.returncode == 0
+    return int([ln for ln in r.stdout.splitlines() if ln.startswith(" Particles:")][0].split()[-1]), \
+        (tmp_path / "o.vtk").read_text()
+
+
+def test_loader_stops_at_first_bad_group(tmp_path):
+    """Stream extraction of 8 fields per particle ends at the first group that
+    does not parse (load_particle.hpp:100-116): later lines are ignored."""
+    n, _ = _count(tmp_path, "1 2 3 4 5 6 7 0\n1 2 x 4 5 6 7 0\n1 2 3 4 5 6 7 1\n")
+    assert n == 1
+
+
+def test_loader_reads_tokens_not_lines(tmp_path):
+    """Seven-column files are misread the way NEXT misreads them: the first
+    token of the next line becomes the type, and the trailing partial group is
+    dropped."""
+    n, vtk = _count(tmp_path, "1 2 3 4 5 6 7\n8 9 10 11 12 13 14\n")
+    assert n == 1
+    assert "SCALARS type int 1\nLOOKUP_TABLE default\n8\n" in vtk
+
+
+def test_missing_file_gives_zero_particles(tmp_path):
+    r = run(tmp_path / "nope.txt", 1, 0.001, 0.1, "vtu", "--dump-ic", tmp_path / "e.vtu")
+    assert r.returncode == 0 and " Particles: 0" in r.stdout
+
+
+def test_ic_text_roundtrip(tmp_path):
+    p = ic.plummer(100, seed=3, type_fraction_dm=0.3)
+    path = tmp_path / "ic.txt"
+    ic.write_ic(path, p)
+    q = ic.read_ic(path)
+    for a, b in zip(p.arrays(), q.arrays()):
+        assert np.array_equal(a, b)
