@@ -67,8 +67,9 @@ typedef struct nbx_config {
   int precision;     /* nbx_precision (NBX_MODE_PHYSICS only) */
   double theta;      /* opening angle; NEXT fixes 0.5 (step.h:211) */
   double eps;        /* Plummer softening length (NBX_MODE_PHYSICS only) */
-  int leaf_size;     /* max bodies per leaf (NBX_MODE_PHYSICS only) */
-  int group_size;    /* targets per walk group (NBX_MODE_PHYSICS only) */
+  int leaf_size;     /* max bodies per leaf bucket, 1..64 (NBX_MODE_PHYSICS only) */
+  int group_size;    /* targets per walk group; 64 (NBX_MODE_PHYSICS only) */
+  int adaptive_dt;   /* nbx_run: 1 = NEXT's speed-adaptive dt (adaptive.h), 0 = fixed dt */
 } nbx_config;
 
 typedef struct nbx_ctx nbx_ctx;
@@ -93,7 +94,8 @@ typedef struct nbx_timing {
   int64_t n_iter;      /* node visits summed over waves (if counting) */
 } nbx_timing;
 
-/* Fill *cfg with NEXT's settings: device 0, reference mode, FP64, theta 0.5. */
+/* Fill *cfg with NEXT's settings: device 0, reference mode, FP64, theta 0.5,
+ * adaptive dt; physics defaults eps 0.01, leaf_size 16, group_size 64. */
 void nbx_config_default(nbx_config* cfg);
 
 nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg);
@@ -127,6 +129,19 @@ nbx_status nbx_run(nbx_ctx* ctx, double base_dt, int32_t nsteps, double* dts_out
 /* Accelerations at the current positions (bhAccel over all targets,
  * octree.h:133-179), upload order. */
 nbx_status nbx_accel(nbx_ctx* ctx, double* ax, double* ay, double* az);
+
+/* NBX_MODE_PHYSICS: gravitational potential of every body at the current
+ * positions from the tree walk (self pair excluded), upload order. */
+nbx_status nbx_potential(nbx_ctx* ctx, double* pot);
+
+/* FP64 direct sum over all pairs with Plummer softening cfg.eps (G = 1):
+ * the force reference for either mode.  Any output may be null. */
+nbx_status nbx_direct(nbx_ctx* ctx, double* ax, double* ay, double* az, double* pot);
+
+/* Energy and momentum of the current state, FP64 with a fixed reduction
+ * order: out5 = {kinetic, potential, px, py, pz}.  source 0: potential from
+ * the tree walk (NBX_MODE_PHYSICS), 1: from the FP64 direct sum. */
+nbx_status nbx_energy(nbx_ctx* ctx, int source, double* out5);
 
 /* Accumulated timings and counters. */
 nbx_status nbx_get_timing(nbx_ctx* ctx, nbx_timing* out);

@@ -35,6 +35,7 @@ EXACT = ["-ffp-contract=off", "-fno-fast-math"]
 SOURCES = [
     ("prims.hip", []),
     ("ref_kernels.hip", EXACT),
+    ("phys_kernels.hip", []),
     ("nbx_api.cpp", []),
 ]
 

@@ -22,6 +22,7 @@
 #include "common.hpp"
 #include "nbx.h"
 #include "prims.hpp"
+#include "phys_kernels.hpp"
 #include "ref_kernels.hpp"
 
 using namespace nbx;
@@ -51,8 +52,10 @@ struct nbx_ctx {
   DBuf nodes;
   DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
   DBuf scratch, bbox_part, dts, lvl_part;
+  DBuf pnodes, paux, pot, epart, eout;   // physics-mode tree, moments, potential, energy
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
+  bool physics = false;       // NBX_MODE_PHYSICS
   bool acc_valid = false;     // acc holds a(x) of the current positions
   bool tree_valid = false;    // nodes hold the tree of the current positions
   const uint64_t* skey = nullptr;
@@ -233,8 +236,19 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
   int64_t off[kMaxDepthSlots + 1];
   off[0] = 0;
   for (int d = 0; d < kMaxDepthSlots; ++d) off[d + 1] = off[d] + c->hsc->level_count[d];
-  for (int d = kMaxDepthSlots - 1; d >= 0; --d)
-    if (c->hsc->level_count[d] > 0) launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, s);
+  if (c->physics) {
+    NBXCHK(ensure(c, c->pnodes, (size_t)total * sizeof(PhysNode)));
+    NBXCHK(ensure(c, c->paux, (size_t)total * 64));
+    launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), S, P<PhysNode>(c->pnodes), c->paux.p, s);
+    for (int d = kMaxDepthSlots - 1; d >= 0; --d)
+      if (c->hsc->level_count[d] > 0)
+        launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<PhysNode>(c->pnodes),
+                           c->paux.p, n, total, c->cfg.theta, s);
+  } else {
+    for (int d = kMaxDepthSlots - 1; d >= 0; --d)
+      if (c->hsc->level_count[d] > 0)
+        launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, s);
+  }
   record(c, EV_UPWARD);
   c->tree_valid = true;
   return NBX_OK;
@@ -245,8 +259,13 @@ nbx_status walk(nbx_ctx* c, bool kick) {
   if (count) launch_fill_scalars(c->sc, 8, c->stream);
   if (kick) launch_fill_scalars(c->sc, 2, c->stream);
   record(c, EV_WALK0);
-  launch_walk(c->n, c->total_nodes, P<RefNode>(c->nodes), view(c, c->cur), c->cfg.theta, P<double>(c->acc[0]),
-              P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
+  if (c->physics)
+    launch_phys_walk(c->n, c->total_nodes, P<PhysNode>(c->pnodes), view(c, c->cur), c->cfg.eps, c->cfg.leaf_size,
+                     P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr, c->sc, kick, count,
+                     c->cfg.precision == NBX_PREC_FP64, c->stream);
+  else
+    launch_walk(c->n, c->total_nodes, P<RefNode>(c->nodes), view(c, c->cur), c->cfg.theta, P<double>(c->acc[0]),
+                P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
   record(c, EV_WALK);
   HIPCHK(c, hipGetLastError());
   c->acc_valid = true;
@@ -335,6 +354,7 @@ void nbx_config_default(nbx_config* cfg) {
   cfg->eps = 0.01;
   cfg->leaf_size = 16;
   cfg->group_size = 64;
+  cfg->adaptive_dt = 1;
 }
 
 nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
@@ -344,7 +364,14 @@ nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
   nbx_config_default(&c0);
   if (cfg) c0 = *cfg;
   if (!(c0.theta >= 0.0)) return NBX_ERR_ARG;
-  if (c0.mode != NBX_MODE_REFERENCE) return NBX_ERR_ARG;
+  if (c0.mode != NBX_MODE_REFERENCE && c0.mode != NBX_MODE_PHYSICS) return NBX_ERR_ARG;
+  if (c0.mode == NBX_MODE_PHYSICS) {
+    // The group test needs theta < 1 (a cell holding a target is then never
+    // accepted for that target's group) and a positive softening length.
+    if (!(c0.theta > 0.0 && c0.theta < 1.0) || !(c0.eps > 0.0)) return NBX_ERR_ARG;
+    if (c0.leaf_size < 1 || c0.leaf_size > kPhysMaxLeaf || c0.group_size != kPhysBlock) return NBX_ERR_ARG;
+    if (c0.precision != NBX_PREC_FP64 && c0.precision != NBX_PREC_MIXED) return NBX_ERR_ARG;
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     (void)hipGetLastError();
@@ -354,6 +381,7 @@ nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
   if (hipSetDevice(c0.device) != hipSuccess) return NBX_ERR_HIP;
   nbx_ctx* c = new nbx_ctx;
   c->cfg = c0;
+  c->physics = c0.mode == NBX_MODE_PHYSICS;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->sc, sizeof(DevScalars)) != hipSuccess ||
       hipHostMalloc(reinterpret_cast<void**>(&c->hsc), sizeof(DevScalars), hipHostMallocDefault) != hipSuccess) {
@@ -383,7 +411,7 @@ void nbx_destroy(nbx_ctx* c) {
   for (auto& b : c->key2) release(b);
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->bbox_part,
-                  &c->dts, &c->lvl_part})
+                  &c->dts, &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->sc) (void)hipFree(c->sc);
@@ -396,6 +424,8 @@ const char* nbx_last_error(const nbx_ctx* c) { return c ? c->err.c_str() : "null
 
 nbx_status nbx_set_theta(nbx_ctx* c, double theta) {
   if (!c || !(theta >= 0.0)) return NBX_ERR_ARG;
+  if (c->physics && !(theta > 0.0 && theta < 1.0)) return fail(c, NBX_ERR_ARG, "physics mode needs 0 < theta < 1");
+  c->tree_valid = false;
   c->cfg.theta = theta;
   c->acc_valid = false;
   return NBX_OK;
@@ -466,16 +496,21 @@ nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) 
   NBXCHK(check_ctx(c));
   if (nsteps < 0) return fail(c, NBX_ERR_ARG, "negative step count");
   if (nsteps == 0) return NBX_OK;
-  if (c->n == 0) {  // computeAdaptiveDt of no particles is base_dt; Step is a no-op
+  if (c->n == 0) {  // computeAdaptiveDt of no particles is base_dt (and so is a fixed dt); Step is a no-op
     if (dts_out) for (int s = 0; s < nsteps; ++s) dts_out[s] = base_dt;
     return NBX_OK;
   }
   NBXCHK(ensure(c, c->dts, (size_t)nsteps * 8));
   launch_fill_scalars(c->sc, 2, c->stream);
   launch_maxv2(c->n, view(c, c->cur), c->sc, c->stream);
+  if (!c->cfg.adaptive_dt) launch_set_dt(c->sc, base_dt, c->stream);
   for (int32_t s = 0; s < nsteps; ++s) {
-    launch_dt(c->sc, base_dt, P<double>(c->dts), s, c->stream);
+    if (c->cfg.adaptive_dt) launch_dt(c->sc, base_dt, P<double>(c->dts), s, c->stream);
     NBXCHK(step_device(c));
+  }
+  if (!c->cfg.adaptive_dt && dts_out) {
+    for (int s = 0; s < nsteps; ++s) dts_out[s] = base_dt;
+    dts_out = nullptr;
   }
   if (dts_out)
     HIPCHK(c, hipMemcpyAsync(dts_out, c->dts.p, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
@@ -501,6 +536,81 @@ nbx_status nbx_accel(nbx_ctx* c, double* ax, double* ay, double* az) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     unpermute(id, tmp.data(), outs[k]);
   }
+  return NBX_OK;
+}
+
+// Copy a sorted-order device array out in upload order.
+static nbx_status download_unpermuted(nbx_ctx* c, const void* dev, double* out) {
+  const int64_t n = c->n;
+  std::vector<int32_t> id((size_t)n);
+  std::vector<double> tmp((size_t)n);
+  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur][8].p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), dev, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  unpermute(id, tmp.data(), out);
+  return NBX_OK;
+}
+
+// Tree potential of the current positions (physics mode).
+static nbx_status tree_potential(nbx_ctx* c) {
+  NBXCHK(ensure(c, c->pot, (size_t)std::max<int64_t>(c->n, 1) * 8));
+  if (!c->tree_valid) {
+    NBXCHK(build_tree(c, false));
+    c->acc_valid = false;
+  }
+  launch_phys_walk(c->n, c->total_nodes, P<PhysNode>(c->pnodes), view(c, c->cur), c->cfg.eps, c->cfg.leaf_size,
+                   P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), P<double>(c->pot), c->sc, false,
+                   false, c->cfg.precision == NBX_PREC_FP64, c->stream);
+  HIPCHK(c, hipGetLastError());
+  c->acc_valid = true;
+  return NBX_OK;
+}
+
+nbx_status nbx_potential(nbx_ctx* c, double* pot) {
+  NBXCHK(check_ctx(c));
+  if (!c->physics) return fail(c, NBX_ERR_ARG, "the tree potential exists in physics mode only");
+  if (c->n == 0) return NBX_OK;
+  if (!pot) return fail(c, NBX_ERR_ARG, "null output");
+  NBXCHK(tree_potential(c));
+  return download_unpermuted(c, c->pot.p, pot);
+}
+
+nbx_status nbx_direct(nbx_ctx* c, double* ax, double* ay, double* az, double* pot) {
+  NBXCHK(check_ctx(c));
+  if (c->n == 0) return NBX_OK;
+  if (!(c->cfg.eps >= 0.0)) return fail(c, NBX_ERR_ARG, "negative softening");
+  DBuf tmp;
+  NBXCHK(ensure(c, tmp, (size_t)c->n * 8 * 4));
+  double* d = P<double>(tmp);
+  launch_direct64(c->n, view(c, c->cur), c->cfg.eps, d, d + c->n, d + 2 * c->n, d + 3 * c->n, c->stream);
+  nbx_status st = NBX_OK;
+  double* outs[4] = {ax, ay, az, pot};
+  for (int k = 0; k < 4 && st == NBX_OK; ++k)
+    if (outs[k]) st = download_unpermuted(c, d + k * c->n, outs[k]);
+  (void)hipStreamSynchronize(c->stream);
+  release(tmp);
+  return st;
+}
+
+nbx_status nbx_energy(nbx_ctx* c, int source, double* out5) {
+  NBXCHK(check_ctx(c));
+  if (!out5) return fail(c, NBX_ERR_ARG, "null output");
+  if (source != 0 && source != 1) return fail(c, NBX_ERR_ARG, "source must be 0 (tree) or 1 (direct)");
+  if (source == 0 && !c->physics) return fail(c, NBX_ERR_ARG, "the tree potential exists in physics mode only");
+  if (c->n == 0) { for (int k = 0; k < 5; ++k) out5[k] = 0.0; return NBX_OK; }
+  NBXCHK(ensure(c, c->pot, (size_t)c->n * 8));
+  NBXCHK(ensure(c, c->epart, (size_t)kEnergyBlocks * 5 * 8));
+  NBXCHK(ensure(c, c->eout, 5 * 8));
+  if (source == 0) {
+    NBXCHK(tree_potential(c));
+  } else {
+    launch_direct64(c->n, view(c, c->cur), c->cfg.eps, P<double>(c->acc[0]), P<double>(c->acc[1]),
+                    P<double>(c->acc[2]), P<double>(c->pot), c->stream);
+    c->acc_valid = false;   // the direct-sum forces replaced the tree forces
+  }
+  launch_energy(c->n, view(c, c->cur), P<double>(c->pot), P<double>(c->epart), P<double>(c->eout), c->stream);
+  HIPCHK(c, hipMemcpyAsync(out5, c->eout.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBX_OK;
 }
 

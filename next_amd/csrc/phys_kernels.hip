@@ -1,0 +1,422 @@
+// Physics-mode kernels on gfx950: a bucket-leaf Barnes-Hut walk with
+// traceless quadrupoles, Plummer softening and a group acceptance test, plus
+// an FP64 direct sum.
+//
+// Tree: the same depth-first preorder octree as reference mode (one body per
+// leaf, skip pointers), read through a compact 64-byte PhysNode.  A node
+// whose body count is at most `nleaf` that must be opened is treated as a
+// bucket: its bodies go straight to the direct-sum list.
+//
+// Multipoles (FP64): m, centre of mass, Q_ij = sum m (3 r_i r_j - r^2 d_ij)
+// about the centre of mass, propagated with the parallel-axis rule
+//   Q = sum_c [Q_c + m_c (3 r_c r_c^T - |r_c|^2 I)],  r_c = com_c - com,
+// and bmax, a bound on the distance from the centre of mass to any body:
+//   bmax = max_c (|r_c| + bmax_c).
+//
+// Walk: one wave = one group of 64 consecutive (Morton-ordered) targets.
+// The wave walks the preorder tree with a decision shared by the whole
+// group: a node is accepted when the distance from its centre of mass to the
+// group's bounding box exceeds bmax/theta.  Node records are read at a
+// wave-uniform index (scalar loads).  Accepted cells and bucket bodies are
+// appended, relative to the group centre, to per-wave LDS lists; a full list
+// is evaluated by all 64 lanes at once (every lane busy, entries read as LDS
+// broadcasts).  Each flush's sum is formed in the evaluation precision and
+// added to FP64 totals.
+//
+// Acceleration on a target at distance vector d = com - x from a cell:
+//   a = m d / r^3 - Q d / r^5 + (5/2) (d^T Q d) d / r^7,  r^2 = |d|^2 + eps^2
+// (the gradient of phi = -m/r - (1/2) x^T Q x / r^5 with x = -d).
+#include "common.hpp"
+#include "phys_kernels.hpp"
+#include "prims.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace nbx {
+namespace {
+
+struct PhysAux {
+  double m, bmax;
+  double q[6];   // xx yy zz xy xz yz
+};
+static_assert(sizeof(PhysAux) == 64, "PhysAux layout");
+
+inline unsigned grid_of(int64_t n, int b) { return (unsigned)div_up(n, b); }
+
+__global__ __launch_bounds__(kBlock) void k_phys_leaves(int64_t n, const int32_t* __restrict__ lcp,
+                                                        const uint32_t* __restrict__ pre_off, StateArrays st,
+                                                        PhysNode* __restrict__ pn, PhysAux* __restrict__ aux) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int li = lcp[i];
+  const int lp = i > 0 ? lcp[i - 1] : -1;
+  const uint32_t p = pre_off[i] + (uint32_t)(li > lp ? li - lp : 0);
+  PhysNode nd;
+  nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
+  nd.m = (float)st.m[i];
+  nd.rc2 = INFINITY;
+  nd.qxx = nd.qyy = nd.qxy = nd.qxz = nd.qyz = 0.f;
+  nd.skip = (int32_t)(p + 1);
+  nd.first = (int32_t)i;
+  nd.count = 1;
+  pn[p] = nd;
+  PhysAux a;
+  a.m = st.m[i];
+  a.bmax = 0.0;
+  for (int k = 0; k < 6; ++k) a.q[k] = 0.0;
+  aux[p] = a;
+}
+
+__global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg,
+                                                        int64_t cnt, const RefNode* __restrict__ rn,
+                                                        PhysNode* __restrict__ pn, PhysAux* __restrict__ aux,
+                                                        int64_t n, int64_t total, double inv_theta) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= cnt) return;
+  const int32_t id = level_list[beg + k];
+  const int32_t end = rn[id].skip;
+  const int32_t first = rn[id].first;
+  double M = 0, sx = 0, sy = 0, sz = 0;
+  int32_t c = id + 1;
+  for (int it = 0; it < 8 && c < end; ++it, c = pn[c].skip) {
+    const double cm = aux[c].m;
+    M += cm;
+    sx += pn[c].cx * cm; sy += pn[c].cy * cm; sz += pn[c].cz * cm;
+  }
+  double cx, cy, cz;
+  if (M > 0) { cx = sx / M; cy = sy / M; cz = sz / M; }
+  else { cx = pn[id + 1].cx; cy = pn[id + 1].cy; cz = pn[id + 1].cz; }
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  double bmax = 0;
+  c = id + 1;
+  for (int it = 0; it < 8 && c < end; ++it, c = pn[c].skip) {
+    const PhysAux& a = aux[c];
+    const double rx = pn[c].cx - cx, ry = pn[c].cy - cy, rz = pn[c].cz - cz;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double mc = a.m;
+    q[0] += a.q[0] + mc * (3 * rx * rx - r2);
+    q[1] += a.q[1] + mc * (3 * ry * ry - r2);
+    q[2] += a.q[2] + mc * (3 * rz * rz - r2);
+    q[3] += a.q[3] + mc * 3 * rx * ry;
+    q[4] += a.q[4] + mc * 3 * rx * rz;
+    q[5] += a.q[5] + mc * 3 * ry * rz;
+    const double b = sqrt(r2) + a.bmax;
+    if (b > bmax) bmax = b;
+  }
+  const int32_t next_first = end < total ? rn[end].first : (int32_t)n;
+  PhysNode nd;
+  nd.cx = cx; nd.cy = cy; nd.cz = cz;
+  nd.m = (float)M;
+  nd.count = next_first - first;
+  // Cells of very few bodies cost more as a multipole than as bodies.
+  const double rc = bmax * inv_theta;
+  nd.rc2 = nd.count < 4 ? INFINITY : (float)(rc * rc * (1.0 + 1e-6));
+  nd.qxx = (float)q[0]; nd.qyy = (float)q[1];
+  nd.qxy = (float)q[3]; nd.qxz = (float)q[4]; nd.qyz = (float)q[5];
+  nd.skip = end;
+  nd.first = first;
+  pn[id] = nd;
+  PhysAux a;
+  a.m = M;
+  a.bmax = bmax;
+  for (int j = 0; j < 6; ++j) a.q[j] = q[j];
+  aux[id] = a;
+}
+
+template <typename T>
+struct alignas(4 * sizeof(T)) V4 {
+  T x, y, z, w;
+};
+
+template <typename T>
+__device__ __forceinline__ T rsq(T x);
+template <>
+__device__ __forceinline__ float rsq<float>(float x) { return __frsqrt_rn(x); }
+template <>
+__device__ __forceinline__ double rsq<double>(double x) { return rsqrt(x); }
+
+__device__ __forceinline__ int64_t xcd_group(int nb) {
+  const int per = (nb + 7) / 8;
+  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+template <typename T, bool KICK, bool POT, bool COUNT>
+__global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t total, const PhysNode* __restrict__ pn,
+                                                          StateArrays st, double eps, int nleaf,
+                                                          double* __restrict__ ax, double* __restrict__ ay,
+                                                          double* __restrict__ az, double* __restrict__ pot,
+                                                          DevScalars* sc) {
+  __shared__ V4<T> ca[kPhysCells];   // cell: relative centre of mass, mass
+  __shared__ V4<T> cb[kPhysCells];   // cell: qxx qyy qxy qxz
+  __shared__ T cq[kPhysCells];       // cell: qyz
+  __shared__ V4<T> pl[kPhysParts];   // body: relative position, mass
+
+  const int lane = threadIdx.x;
+  const int64_t g0 = xcd_group(gridDim.x) * kPhysBlock;
+  if (g0 >= n) return;                                     // whole wave past the end
+  const int64_t i = g0 + lane;
+  const bool valid = i < n;
+  const int64_t ii = valid ? i : g0;
+  const double px = st.x[ii], py = st.y[ii], pz = st.z[ii];
+
+  // group bounding box
+  double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double a = __shfl_xor(lo[k], off, kWave), b = __shfl_xor(hi[k], off, kWave);
+      lo[k] = a < lo[k] ? a : lo[k];
+      hi[k] = b > hi[k] ? b : hi[k];
+    }
+  }
+  const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]), gcz = 0.5 * (lo[2] + hi[2]);
+  const float hx = (float)(0.5 * (hi[0] - lo[0])) * (1.0f + 1e-6f);
+  const float hy = (float)(0.5 * (hi[1] - lo[1])) * (1.0f + 1e-6f);
+  const float hz = (float)(0.5 * (hi[2] - lo[2])) * (1.0f + 1e-6f);
+  const T tx = (T)(px - gcx), ty = (T)(py - gcy), tz = (T)(pz - gcz);
+  const T e2 = (T)(eps * eps);
+
+  double Ax = 0, Ay = 0, Az = 0, P = 0;
+  uint32_t ncell_tot = 0, npart_tot = 0;
+  int nc = 0, np = 0;
+
+  auto flush_cells = [&]() {
+    __syncthreads();
+    T fx = 0, fy = 0, fz = 0, fp = 0;
+    for (int k = 0; k < nc; ++k) {
+      const V4<T> a = ca[k];
+      const V4<T> b = cb[k];
+      const T qyz = cq[k];
+      const T dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+      const T r2 = dx * dx + dy * dy + dz * dz + e2;
+      const T ri = rsq<T>(r2);
+      const T ri2 = ri * ri;
+      const T ri3 = ri2 * ri;
+      const T ri5 = ri3 * ri2;
+      const T qzz = -(b.x + b.y);
+      const T qdx = b.x * dx + b.z * dy + b.w * dz;
+      const T qdy = b.z * dx + b.y * dy + qyz * dz;
+      const T qdz = b.w * dx + qyz * dy + qzz * dz;
+      const T dqd = dx * qdx + dy * qdy + dz * qdz;
+      const T mono = a.w * ri3;
+      const T c7 = (T)2.5 * dqd * ri5 * ri2;
+      fx += (mono + c7) * dx - ri5 * qdx;
+      fy += (mono + c7) * dy - ri5 * qdy;
+      fz += (mono + c7) * dz - ri5 * qdz;
+      if (POT) fp -= a.w * ri + (T)0.5 * dqd * ri5;
+    }
+    Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
+    if (POT) P += (double)fp;
+    if (COUNT) ncell_tot += nc;
+    __syncthreads();
+    nc = 0;
+  };
+  auto flush_parts = [&]() {
+    __syncthreads();
+    T fx = 0, fy = 0, fz = 0, fp = 0;
+    for (int k = 0; k < np; ++k) {
+      const V4<T> b = pl[k];
+      const T dx = b.x - tx, dy = b.y - ty, dz = b.z - tz;
+      const T r2 = dx * dx + dy * dy + dz * dz + e2;
+      const T ri = rsq<T>(r2);
+      const T mr3 = b.w * ri * ri * ri;
+      fx += mr3 * dx;
+      fy += mr3 * dy;
+      fz += mr3 * dz;
+      if (POT) fp -= b.w * ri;
+    }
+    Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
+    if (POT) P += (double)fp;
+    if (COUNT) npart_tot += np;
+    __syncthreads();
+    np = 0;
+  };
+
+  int32_t cur = 0;
+  while (cur < total) {
+    cur = __builtin_amdgcn_readfirstlane(cur);
+    const PhysNode* nd = pn + cur;
+    const double ncx = nd->cx, ncy = nd->cy, ncz = nd->cz;
+    const int32_t skip = nd->skip > cur ? nd->skip : cur + 1;
+    const float rx = (float)(ncx - gcx), ry = (float)(ncy - gcy), rz = (float)(ncz - gcz);
+    const float dx = fmaxf(fabsf(rx) - hx, 0.f), dy = fmaxf(fabsf(ry) - hy, 0.f), dz = fmaxf(fabsf(rz) - hz, 0.f);
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const int accepted = __builtin_amdgcn_readfirstlane(d2 > nd->rc2 ? 1 : 0);
+    if (accepted) {                                        // accepted: one multipole
+      if (lane == 0) {
+        ca[nc] = V4<T>{(T)(ncx - gcx), (T)(ncy - gcy), (T)(ncz - gcz), (T)nd->m};
+        cb[nc] = V4<T>{(T)nd->qxx, (T)nd->qyy, (T)nd->qxy, (T)nd->qxz};
+        cq[nc] = (T)nd->qyz;
+      }
+      if (++nc == kPhysCells) flush_cells();
+      cur = skip;
+    } else if (nd->count <= nleaf) {                       // bucket: its bodies
+      const int c = nd->count;
+      if (np + c > kPhysParts) flush_parts();
+      if (lane < c) {
+        const int32_t b = nd->first + lane;
+        pl[np + lane] = V4<T>{(T)(st.x[b] - gcx), (T)(st.y[b] - gcy), (T)(st.z[b] - gcz), (T)st.m[b]};
+      }
+      np += c;
+      cur = skip;
+    } else {
+      cur = cur + 1;                                       // open
+    }
+  }
+  if (nc) flush_cells();
+  if (np) flush_parts();
+
+  double v2 = 0.0;
+  if (valid) {
+    ax[i] = Ax; ay[i] = Ay; az[i] = Az;
+    if (POT) pot[i] = P + st.m[i] / eps;                   // drop the self pair, -m_i/eps
+    if (KICK) {
+      const double half = sc->dt * 0.5;
+      const double vx = st.vx[i] + Ax * half;
+      const double vy = st.vy[i] + Ay * half;
+      const double vz = st.vz[i] + Az * half;
+      st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
+      v2 = vx * vx + vy * vy + vz * vz;
+      if (!(v2 > 0.0)) v2 = 0.0;
+    }
+  }
+  if (KICK) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double o = __shfl_xor(v2, off, kWave);
+      if (o > v2) v2 = o;
+    }
+    if (lane == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
+  }
+  if (COUNT && lane == 0) {
+    const uint32_t nv = (uint32_t)(n - g0 < kPhysBlock ? n - g0 : kPhysBlock);
+    atomicAdd(&sc->n_pc, (unsigned long long)ncell_tot * nv);
+    atomicAdd(&sc->n_pp, (unsigned long long)npart_tot * nv);
+  }
+}
+
+// FP64 direct sum: each thread owns one target; source bodies are staged
+// through LDS in tiles of one block.
+__global__ __launch_bounds__(kBlock) void k_direct64(int64_t n, StateArrays st, double e2, double* __restrict__ ax,
+                                                     double* __restrict__ ay, double* __restrict__ az,
+                                                     double* __restrict__ pot) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sm[kBlock];
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < n;
+  const double px = valid ? st.x[i] : 0, py = valid ? st.y[i] : 0, pz = valid ? st.z[i] : 0;
+  double a0 = 0, a1 = 0, a2 = 0, p = 0;
+  for (int64_t t = 0; t < n; t += kBlock) {
+    const int64_t j = t + threadIdx.x;
+    sx[threadIdx.x] = j < n ? st.x[j] : 0;
+    sy[threadIdx.x] = j < n ? st.y[j] : 0;
+    sz[threadIdx.x] = j < n ? st.z[j] : 0;
+    sm[threadIdx.x] = j < n ? st.m[j] : 0;
+    __syncthreads();
+    const int lim = (int)(n - t < kBlock ? n - t : kBlock);
+    for (int k = 0; k < lim; ++k) {
+      if (t + k == i) continue;
+      const double dx = sx[k] - px, dy = sy[k] - py, dz = sz[k] - pz;
+      const double r2 = dx * dx + dy * dy + dz * dz + e2;
+      const double ri = 1.0 / sqrt(r2);
+      const double mr3 = sm[k] * ri * ri * ri;
+      a0 += dx * mr3; a1 += dy * mr3; a2 += dz * mr3;
+      p -= sm[k] * ri;
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    ax[i] = a0; ay[i] = a1; az[i] = a2;
+    if (pot) pot[i] = p;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_partial(int64_t n, StateArrays st, const double* __restrict__ pot,
+                                                           double* __restrict__ part) {
+  __shared__ double red[kBlock / kWave][5];
+  double v[5] = {0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double m = st.m[i], vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
+    v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    if (pot) v[1] += 0.5 * m * pot[i];
+    v[2] += m * vx; v[3] += m * vy; v[4] += m * vz;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int k = 0; k < 5; ++k) v[k] += __shfl_xor(v[k], off, kWave);
+  if (lane_id() == 0)
+    for (int k = 0; k < 5; ++k) red[threadIdx.x / kWave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += red[w][threadIdx.x];
+    part[5 * blockIdx.x + threadIdx.x] = t;
+  }
+}
+
+__global__ void k_energy_final(const double* __restrict__ part, int nparts, double* out5) {
+  if (threadIdx.x < 5) {
+    double t = 0;
+    for (int b = 0; b < nparts; ++b) t += part[5 * b + threadIdx.x];
+    out5[threadIdx.x] = t;
+  }
+}
+
+template <typename T>
+void walk_dispatch(unsigned nb, int64_t n, int32_t t, const PhysNode* pn, const StateArrays& st, double eps,
+                   int nleaf, double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count,
+                   hipStream_t s) {
+#define NBX_PW(K, P, C)                                                                                          \
+  hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, n, t, pn, st, eps, nleaf, ax, \
+                     ay, az, pot, sc)
+  const bool p = pot != nullptr;
+  if (kick) {
+    if (count) NBX_PW(true, false, true); else NBX_PW(true, false, false);
+  } else if (p) {
+    if (count) NBX_PW(false, true, true); else NBX_PW(false, true, false);
+  } else {
+    if (count) NBX_PW(false, false, true); else NBX_PW(false, false, false);
+  }
+#undef NBX_PW
+}
+
+}  // namespace
+
+void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st, PhysNode* pn,
+                        void* aux, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_phys_leaves, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, st, pn,
+                       reinterpret_cast<PhysAux*>(aux));
+}
+
+void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
+                        void* aux, int64_t n, int64_t total, double theta, hipStream_t s) {
+  if (cnt > 0)
+    hipLaunchKernelGGL(k_phys_upward, dim3(grid_of(cnt, kBlock)), dim3(kBlock), 0, s, level_list, beg, cnt, rn, pn,
+                       reinterpret_cast<PhysAux*>(aux), n, total, 1.0 / theta);
+}
+
+void launch_phys_walk(int64_t n, int64_t total, const PhysNode* pn, const StateArrays& st, double eps, int nleaf,
+                      double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count, bool fp64,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned nb = (unsigned)(div_up(div_up(n, kPhysBlock), 8) * 8);
+  if (fp64)
+    walk_dispatch<double>(nb, n, (int32_t)total, pn, st, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
+  else
+    walk_dispatch<float>(nb, n, (int32_t)total, pn, st, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
+}
+
+void launch_direct64(int64_t n, const StateArrays& st, double eps, double* ax, double* ay, double* az, double* pot,
+                     hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_direct64, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, st, eps * eps, ax, ay, az, pot);
+}
+
+void launch_energy(int64_t n, const StateArrays& st, const double* pot, double* part, double* out5, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_partial, dim3(kEnergyBlocks), dim3(kBlock), 0, s, n, st, pot, part);
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kWave), 0, s, part, kEnergyBlocks, out5);
+}
+
+}  // namespace nbx

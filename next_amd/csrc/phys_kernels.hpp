@@ -1,0 +1,54 @@
+// Launchers for the physics-mode kernels (phys_kernels.hip): bucket-leaf
+// group walk with traceless quadrupoles and Plummer softening, and the FP64
+// direct sum used for force errors and energies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "ref_kernels.hpp"
+
+namespace nbx {
+
+// Walk record of a node (64 bytes): FP64 centre of mass for the
+// group-relative conversion, FP32 moments, and the traversal links.
+struct alignas(64) PhysNode {
+  double cx, cy, cz;
+  float m;
+  float rc2;                  // (bmax / theta)^2; +inf: never accepted as a cell
+  float qxx, qyy, qxy, qxz, qyz;   // traceless: qzz = -(qxx + qyy)
+  int32_t skip;
+  int32_t first;              // first body (sorted index)
+  int32_t count;              // bodies in the subtree
+};
+static_assert(sizeof(PhysNode) == 64, "PhysNode layout");
+
+constexpr int kPhysBlock = 64;      // one wave = one group of targets
+constexpr int kPhysCells = 128;     // per-wave LDS cell list
+constexpr int kPhysParts = 256;     // per-wave LDS body list
+constexpr int kPhysMaxLeaf = 64;    // bucket size limit (one body per lane)
+
+// Leaves: one PhysNode per body.
+// `aux`: 64 bytes per node of FP64 moments (mass, bmax, quadrupole).
+void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st,
+                        PhysNode* pn, void* aux, hipStream_t s);
+// Internal nodes of one depth from their children (FP64 accumulation).
+void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
+                        void* aux, int64_t n, int64_t total, double theta, hipStream_t s);
+// Group walk: accelerations (and potentials if pot != null), optionally
+// followed by the closing half kick.  fp64 selects the FP64 evaluation.
+void launch_phys_walk(int64_t n, int64_t total, const PhysNode* pn, const StateArrays& st, double eps, int nleaf,
+                      double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count,
+                      bool fp64, hipStream_t s);
+// FP64 direct sum with Plummer softening over all pairs (G = 1).
+void launch_direct64(int64_t n, const StateArrays& st, double eps, double* ax, double* ay, double* az, double* pot,
+                     hipStream_t s);
+// Kinetic energy, potential energy (0.5 sum m phi) and momentum, reduced in
+// a fixed order: out5 (device) = {ekin, epot, px, py, pz}.  part holds
+// 5 * kEnergyBlocks doubles.
+constexpr int kEnergyBlocks = 512;
+void launch_energy(int64_t n, const StateArrays& st, const double* pot, double* part, double* out5, hipStream_t s);
+
+}  // namespace nbx

@@ -24,7 +24,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
-           "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree"]
+           "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
+           "nbx_potential", "nbx_direct", "nbx_energy"]
 
 
 class NbxError(RuntimeError):
@@ -35,7 +36,7 @@ class NbxError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int), ("mode", C.c_int), ("precision", C.c_int), ("theta", C.c_double),
-                ("eps", C.c_double), ("leaf_size", C.c_int), ("group_size", C.c_int)]
+                ("eps", C.c_double), ("leaf_size", C.c_int), ("group_size", C.c_int), ("adaptive_dt", C.c_int)]
 
 
 class Timing(C.Structure):
@@ -82,6 +83,9 @@ def lib():
     L.nbx_debug_keys.argtypes = [VP, f64p, u64p, u64p]
     L.nbx_debug_tree_size.argtypes = [VP, C.POINTER(C.c_int64)]
     L.nbx_debug_tree.argtypes = [VP, i32p, i32p, i32p, i32p, f64p, f64p, f64p, f64p, i32p]
+    L.nbx_potential.argtypes = [VP, f64p]
+    L.nbx_direct.argtypes = [VP, VP, VP, VP, VP]
+    L.nbx_energy.argtypes = [VP, C.c_int, f64p]
     _lib = L
     return L
 
@@ -94,12 +98,14 @@ class Engine:
     """One device-resident particle system (an nbx_ctx)."""
 
     def __init__(self, theta: float = 0.5, device: int = 0, mode: int = MODE_REFERENCE,
-                 precision: int = PREC_FP64, eps: float = 0.01, leaf_size: int = 16, group_size: int = 64):
+                 precision: int = PREC_FP64, eps: float = 0.01, leaf_size: int = 16, group_size: int = 64,
+                 adaptive_dt: bool = True):
         L = lib()
         cfg = Config()
         L.nbx_config_default(C.byref(cfg))
         cfg.device, cfg.mode, cfg.precision = device, mode, precision
         cfg.theta, cfg.eps, cfg.leaf_size, cfg.group_size = theta, eps, leaf_size, group_size
+        cfg.adaptive_dt = 1 if adaptive_dt else 0
         h = VP()
         st = L.nbx_create(C.byref(h), C.byref(cfg))
         if st != 0:
@@ -171,6 +177,24 @@ class Engine:
         t = Timing()
         self._check(lib().nbx_get_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def potential(self):
+        out = np.zeros(max(self.n, 1))
+        self._check(lib().nbx_potential(self._h, out))
+        return out[: self.n]
+
+    def direct(self):
+        """FP64 direct-sum accelerations and potentials (Plummer eps)."""
+        out = [np.zeros(max(self.n, 1)) for _ in range(4)]
+        ptrs = [o.ctypes.data_as(C.c_void_p) for o in out]
+        self._check(lib().nbx_direct(self._h, *ptrs))
+        return np.stack([o[: self.n] for o in out[:3]], axis=1), out[3][: self.n]
+
+    def energy(self, direct: bool = False):
+        """(kinetic, potential, px, py, pz) of the current state."""
+        out = np.zeros(5)
+        self._check(lib().nbx_energy(self._h, 1 if direct else 0, out))
+        return out
 
     def debug_keys(self):
         root = np.zeros(4)

@@ -1,0 +1,107 @@
+"""Physics mode on the GPU: FP64 direct sum against the oracle, the
+quadrupole term against brute force, force errors of the FP32 and FP64 group
+walks against the FP64 direct sum, and energy / momentum behaviour.
+
+Tolerances are stated per test; force errors are relative, |a - a_ref| /
+|a_ref| per body, reported as median and 99th percentile."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from next_amd import ic
+from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Engine, NbxError
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    d = np.linalg.norm(b, axis=1)
+    return np.linalg.norm(a - b, axis=1) / np.where(d > 0, d, 1.0)
+
+
+def phys(theta=0.5, eps=0.01, precision=PREC_MIXED, leaf=16, adaptive=False):
+    return Engine(theta=theta, mode=MODE_PHYSICS, precision=precision, eps=eps, leaf_size=leaf,
+                  adaptive_dt=adaptive)
+
+
+def test_direct_sum_matches_oracle():
+    p = ic.plummer(3000, seed=21, virial=None)
+    with phys(eps=0.01) as e:
+        e.upload(p)
+        a, pot = e.direct()
+    ao, po = orc.direct(p.x, p.y, p.z, p.m, 0.01)
+    assert np.max(rel_err(a, ao)) <= 1e-12
+    assert np.max(np.abs(pot - po) / np.abs(po)) <= 1e-12
+
+
+def test_quadrupole_term_against_brute_force():
+    """A compact cluster seen from afar is accepted as one cell: with the
+    quadrupole (right sign) the error is third order in size/distance; without
+    it, or with the sign flipped, it is second order."""
+    rng = np.random.Generator(np.random.PCG64(4))
+    cl = rng.uniform(-0.3, 0.3, (64, 3)) * np.array([1.0, 0.4, 0.2])
+    far = np.array([[4.0, 2.5, 1.0]])
+    pos = np.concatenate([cl, far])
+    m = np.concatenate([rng.uniform(0.5, 1.5, 64), [1e-9]])
+    p = ic.Particles.from_columns(np.concatenate([pos, np.zeros_like(pos), m[:, None]], 1),
+                                  np.zeros(65, np.int32))
+    with phys(theta=0.5, eps=1e-6, precision=PREC_FP64, leaf=4) as e:
+        e.upload(p)
+        a = e.accel()[-1]
+        ad, _ = e.direct()
+    ref = ad[-1]
+    mono = (m[:64, None] * (cl - far)).sum(0) / np.linalg.norm(cl.mean(0) - far) ** 3
+    err = np.linalg.norm(a - ref) / np.linalg.norm(ref)
+    err_mono = np.linalg.norm(mono - ref) / np.linalg.norm(ref)
+    print(f"cell far field: quadrupole err {err:.2e}, monopole-only err {err_mono:.2e}")
+    assert err < 0.2 * err_mono and err < 2e-3
+
+
+@pytest.mark.parametrize("precision,theta,med_tol,p99_tol", [
+    (PREC_FP64, 0.3, 6e-5, 6e-4),
+    (PREC_MIXED, 0.3, 6e-5, 6e-4),
+    (PREC_MIXED, 0.5, 6e-4, 6e-3),
+    (PREC_MIXED, 0.7, 4e-3, 3e-2),
+])
+def test_group_walk_force_error(precision, theta, med_tol, p99_tol):
+    p = ic.plummer(32768, seed=31)
+    with phys(theta=theta, eps=0.01, precision=precision) as e:
+        e.upload(p)
+        a = e.accel()
+        ad, _ = e.direct()
+    err = rel_err(a, ad)
+    med, p99 = float(np.median(err)), float(np.percentile(err, 99))
+    print(f"prec={precision} theta={theta}: median {med:.2e} p99 {p99:.2e}")
+    assert med <= med_tol and p99 <= p99_tol
+
+
+def test_energy_and_momentum_over_steps():
+    p = ic.plummer(16384, seed=1)
+    with phys(theta=0.5, eps=0.01, precision=PREC_MIXED) as e:
+        e.upload(p)
+        e0 = e.energy(direct=True)
+        e.run(1e-3, 100)
+        e1 = e.energy(direct=True)
+    E0, E1 = e0[0] + e0[1], e1[0] + e1[1]
+    dE = abs(E1 - E0) / abs(E0)
+    dp = np.linalg.norm(e1[2:] - e0[2:])
+    print(f"100 steps: |dE/E| {dE:.2e}, |dP| {dp:.2e}")
+    assert dE <= 1e-4 and dp <= 1e-5
+
+
+def test_tree_potential_close_to_direct():
+    p = ic.plummer(8192, seed=5)
+    with phys(theta=0.4, eps=0.01, precision=PREC_FP64) as e:
+        e.upload(p)
+        pt = e.potential()
+        _, pd = e.direct()
+    err = np.abs(pt - pd) / np.abs(pd)
+    assert np.median(err) <= 1e-5
+
+
+def test_physics_mode_rejects_bad_settings():
+    for kw in (dict(theta=1.2), dict(eps=0.0), dict(leaf=0), dict(leaf=65)):
+        args = dict(theta=0.5, eps=0.01, leaf=16)
+        args.update(kw)
+        with pytest.raises(NbxError):
+            phys(**args)
