@@ -238,7 +238,7 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
   for (int d = 0; d < kMaxDepthSlots; ++d) off[d + 1] = off[d] + c->hsc->level_count[d];
   if (c->physics) {
     NBXCHK(ensure(c, c->pnodes, (size_t)total * sizeof(PhysNode)));
-    NBXCHK(ensure(c, c->paux, (size_t)total * 64));
+    NBXCHK(ensure(c, c->paux, (size_t)total * kPhysAuxBytes));
     launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), S, P<PhysNode>(c->pnodes), c->paux.p, s);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)

@@ -38,9 +38,11 @@ namespace {
 
 struct PhysAux {
   double m, bmax;
-  double q[6];   // xx yy zz xy xz yz
+  double q[6];          // xx yy zz xy xz yz
+  double lo[3], hi[3];  // bounding box of the node's bodies
+  double pad[2];
 };
-static_assert(sizeof(PhysAux) == 64, "PhysAux layout");
+static_assert(sizeof(PhysAux) == kPhysAuxBytes, "PhysAux layout");
 
 inline unsigned grid_of(int64_t n, int b) { return (unsigned)div_up(n, b); }
 
@@ -65,6 +67,8 @@ __global__ __launch_bounds__(kBlock) void k_phys_leaves(int64_t n, const int32_t
   a.m = st.m[i];
   a.bmax = 0.0;
   for (int k = 0; k < 6; ++k) a.q[k] = 0.0;
+  a.lo[0] = a.hi[0] = nd.cx; a.lo[1] = a.hi[1] = nd.cy; a.lo[2] = a.hi[2] = nd.cz;
+  a.pad[0] = a.pad[1] = 0.0;
   aux[p] = a;
 }
 
@@ -88,7 +92,8 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
   if (M > 0) { cx = sx / M; cy = sy / M; cz = sz / M; }
   else { cx = pn[id + 1].cx; cy = pn[id + 1].cy; cz = pn[id + 1].cz; }
   double q[6] = {0, 0, 0, 0, 0, 0};
-  double bmax = 0;
+  double bsum = 0;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   c = id + 1;
   for (int it = 0; it < 8 && c < end; ++it, c = pn[c].skip) {
     const PhysAux& a = aux[c];
@@ -102,8 +107,17 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
     q[4] += a.q[4] + mc * 3 * rx * rz;
     q[5] += a.q[5] + mc * 3 * ry * rz;
     const double b = sqrt(r2) + a.bmax;
-    if (b > bmax) bmax = b;
+    if (b > bsum) bsum = b;
+    for (int d = 0; d < 3; ++d) {
+      if (a.lo[d] < lo[d]) lo[d] = a.lo[d];
+      if (a.hi[d] > hi[d]) hi[d] = a.hi[d];
+    }
   }
+  // bmax: the farthest corner of the bodies' box from the centre of mass,
+  // or the children's bound if that is smaller.
+  const double ex = fmax(cx - lo[0], hi[0] - cx), ey = fmax(cy - lo[1], hi[1] - cy), ez = fmax(cz - lo[2], hi[2] - cz);
+  double bmax = sqrt(ex * ex + ey * ey + ez * ez);
+  if (bsum < bmax) bmax = bsum;
   const int32_t next_first = end < total ? rn[end].first : (int32_t)n;
   PhysNode nd;
   nd.cx = cx; nd.cy = cy; nd.cz = cz;
@@ -121,6 +135,8 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
   a.m = M;
   a.bmax = bmax;
   for (int j = 0; j < 6; ++j) a.q[j] = q[j];
+  for (int d = 0; d < 3; ++d) { a.lo[d] = lo[d]; a.hi[d] = hi[d]; }
+  a.pad[0] = a.pad[1] = 0.0;
   aux[id] = a;
 }
 
@@ -141,16 +157,30 @@ __device__ __forceinline__ int64_t xcd_group(int nb) {
   return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// Group walk with work donation.  Every lane walks a contiguous range of
+// the preorder node array with skip pointers (accept or bucket: jump past the
+// subtree; open: step to the first child).  Lane 0 starts with the whole
+// tree; when a lane opens a node, it hands the rest of its range after that
+// node's subtree, [skip, end), to a per-wave LDS stack from which idle lanes
+// take work (last in, first out).  Handing work over is optional, so a full
+// stack only means a lane keeps walking its range itself: the stack cannot
+// overflow.  All decisions are made in lockstep by one wave and compacted in
+// lane order, so the lists, and every sum, come out in the same order on
+// every run.
 template <typename T, bool KICK, bool POT, bool COUNT>
 __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t total, const PhysNode* __restrict__ pn,
                                                           StateArrays st, double eps, int nleaf,
                                                           double* __restrict__ ax, double* __restrict__ ay,
                                                           double* __restrict__ az, double* __restrict__ pot,
                                                           DevScalars* sc) {
-  __shared__ V4<T> ca[kPhysCells];   // cell: relative centre of mass, mass
-  __shared__ V4<T> cb[kPhysCells];   // cell: qxx qyy qxy qxz
-  __shared__ T cq[kPhysCells];       // cell: qyz
-  __shared__ V4<T> pl[kPhysParts];   // body: relative position, mass
+  __shared__ int32_t stk_b[kPhysStack];  // donated ranges [stk_b, stk_e)
+  __shared__ int32_t stk_e[kPhysStack];
+  __shared__ int32_t bfirst[kWave];      // buckets of the current step: first body,
+  __shared__ int32_t bstart[kWave];      //   offset in the step's flattened body list
+  __shared__ V4<T> ca[kPhysCells];       // cell: relative centre of mass, mass
+  __shared__ V4<T> cb[kPhysCells];       // cell: qxx qyy qxy qxz
+  __shared__ T cq[kPhysCells];           // cell: qyz
+  __shared__ V4<T> pl[kPhysParts];       // body: relative position, mass
 
   const int lane = threadIdx.x;
   const int64_t g0 = xcd_group(gridDim.x) * kPhysBlock;
@@ -159,6 +189,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t tot
   const bool valid = i < n;
   const int64_t ii = valid ? i : g0;
   const double px = st.x[ii], py = st.y[ii], pz = st.z[ii];
+  const uint64_t ltmask = lanemask_lt();
 
   // group bounding box
   double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
@@ -234,36 +265,86 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t tot
     np = 0;
   };
 
-  int32_t cur = 0;
-  while (cur < total) {
-    cur = __builtin_amdgcn_readfirstlane(cur);
-    const PhysNode* nd = pn + cur;
-    const double ncx = nd->cx, ncy = nd->cy, ncz = nd->cz;
-    const int32_t skip = nd->skip > cur ? nd->skip : cur + 1;
-    const float rx = (float)(ncx - gcx), ry = (float)(ncy - gcy), rz = (float)(ncz - gcz);
-    const float dx = fmaxf(fabsf(rx) - hx, 0.f), dy = fmaxf(fabsf(ry) - hy, 0.f), dz = fmaxf(fabsf(rz) - hz, 0.f);
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    const int accepted = __builtin_amdgcn_readfirstlane(d2 > nd->rc2 ? 1 : 0);
-    if (accepted) {                                        // accepted: one multipole
-      if (lane == 0) {
-        ca[nc] = V4<T>{(T)(ncx - gcx), (T)(ncy - gcy), (T)(ncz - gcz), (T)nd->m};
-        cb[nc] = V4<T>{(T)nd->qxx, (T)nd->qyy, (T)nd->qxy, (T)nd->qxz};
-        cq[nc] = (T)nd->qyz;
-      }
-      if (++nc == kPhysCells) flush_cells();
-      cur = skip;
-    } else if (nd->count <= nleaf) {                       // bucket: its bodies
-      const int c = nd->count;
-      if (np + c > kPhysParts) flush_parts();
-      if (lane < c) {
-        const int32_t b = nd->first + lane;
+  int32_t cur = lane == 0 ? 0 : total;                     // this lane's range [cur, end)
+  int32_t end = total;
+  int sp = 0;                                              // donated ranges on the stack
+  for (;;) {
+    // idle lanes take ranges from the top of the stack
+    const bool idle = cur >= end;
+    const uint64_t im = __ballot(idle);
+    const int take = __popcll(im) < sp ? (int)__popcll(im) : sp;
+    if (idle) {
+      const int r = (int)__popcll(im & ltmask);
+      if (r < take) { cur = stk_b[sp - 1 - r]; end = stk_e[sp - 1 - r]; }
+    }
+    sp -= take;
+    const bool busy = cur < end;
+    if (!__ballot(busy)) break;
+    if (nc > kPhysCells - kWave) flush_cells();
+
+    bool acc = false, bucket = false, donate = false;
+    int32_t cnt = 0, first = 0, nskip = 0;
+    double ncx = 0, ncy = 0, ncz = 0;
+    if (busy) {
+      const PhysNode nd = pn[cur];
+      ncx = nd.cx; ncy = nd.cy; ncz = nd.cz;
+      const float rx = (float)(ncx - gcx), ry = (float)(ncy - gcy), rz = (float)(ncz - gcz);
+      const float dx = fmaxf(fabsf(rx) - hx, 0.f), dy = fmaxf(fabsf(ry) - hy, 0.f), dz = fmaxf(fabsf(rz) - hz, 0.f);
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      cnt = nd.count;
+      first = nd.first;
+      nskip = nd.skip > cur ? nd.skip : cur + 1;           // skip > cur by construction
+      if (d2 > nd.rc2) acc = true;
+      else if (cnt <= nleaf) bucket = true;
+      else donate = nskip < end;                           // open; the rest of the range can go
+    }
+    // accepted cells -> cell list
+    const uint64_t am = __ballot(acc);
+    if (acc) {
+      const int pos = nc + (int)__popcll(am & ltmask);
+      const PhysNode& nd = pn[cur];
+      ca[pos] = V4<T>{(T)(ncx - gcx), (T)(ncy - gcy), (T)(ncz - gcz), (T)nd.m};
+      cb[pos] = V4<T>{(T)nd.qxx, (T)nd.qyy, (T)nd.qxy, (T)nd.qxz};
+      cq[pos] = (T)nd.qyz;
+    }
+    nc += (int)__popcll(am);
+    // opened nodes hand [skip, end) to the stack while it has room
+    const uint64_t dm = __ballot(donate);
+    const int room = kPhysStack - sp;
+    if (donate) {
+      const int r = (int)__popcll(dm & ltmask);
+      if (r < room) { stk_b[sp + r] = nskip; stk_e[sp + r] = end; end = nskip; }
+    }
+    sp += (int)__popcll(dm) < room ? (int)__popcll(dm) : room;
+    if (busy) cur = (acc || bucket) ? nskip : cur + 1;
+
+    // buckets -> body list, all lanes loading bodies in parallel
+    const uint32_t cb_n = bucket ? (uint32_t)cnt : 0u;
+    const uint32_t bincl = wave_incl_scan_u32(cb_n);
+    const int nbody = (int)__shfl(bincl, kWave - 1, kWave);
+    const uint64_t bm = __ballot(bucket);
+    const int nb = (int)__popcll(bm);
+    if (bucket) {
+      const int j = (int)__popcll(bm & ltmask);
+      bfirst[j] = first;
+      bstart[j] = (int)(bincl - cb_n);
+    }
+    __syncthreads();
+    for (int base = 0; base < nbody; base += kWave) {
+      if (np > kPhysParts - kWave) flush_parts();
+      const int t = base + lane;
+      if (t < nbody) {
+        int lo_j = 0, hi_j = nb - 1;                       // last bucket with bstart <= t
+        while (lo_j < hi_j) {
+          const int mid = (lo_j + hi_j + 1) >> 1;
+          if (bstart[mid] <= t) lo_j = mid; else hi_j = mid - 1;
+        }
+        const int32_t b = bfirst[lo_j] + (t - bstart[lo_j]);
         pl[np + lane] = V4<T>{(T)(st.x[b] - gcx), (T)(st.y[b] - gcy), (T)(st.z[b] - gcz), (T)st.m[b]};
       }
-      np += c;
-      cur = skip;
-    } else {
-      cur = cur + 1;                                       // open
+      np += nbody - base < kWave ? nbody - base : kWave;
     }
+    __syncthreads();
   }
   if (nc) flush_cells();
   if (np) flush_parts();
@@ -367,7 +448,7 @@ template <typename T>
 void walk_dispatch(unsigned nb, int64_t n, int32_t t, const PhysNode* pn, const StateArrays& st, double eps,
                    int nleaf, double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count,
                    hipStream_t s) {
-#define NBX_PW(K, P, C)                                                                                          \
+#define NBX_PW(K, P, C)                                                                                       \
   hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, n, t, pn, st, eps, nleaf, ax, \
                      ay, az, pot, sc)
   const bool p = pot != nullptr;

@@ -29,9 +29,12 @@ constexpr int kPhysBlock = 64;      // one wave = one group of targets
 constexpr int kPhysCells = 128;     // per-wave LDS cell list
 constexpr int kPhysParts = 256;     // per-wave LDS body list
 constexpr int kPhysMaxLeaf = 64;    // bucket size limit (one body per lane)
+constexpr int kPhysStack = 256;     // per-wave LDS stack of donated node ranges
+constexpr int kPhysAuxBytes = 128;  // FP64 moments per node (mass, bmax, Q, box)
 
 // Leaves: one PhysNode per body.
-// `aux`: 64 bytes per node of FP64 moments (mass, bmax, quadrupole).
+// `aux`: kPhysAuxBytes per node of FP64 moments (mass, bmax, quadrupole,
+// bounding box of the bodies).
 void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st,
                         PhysNode* pn, void* aux, hipStream_t s);
 // Internal nodes of one depth from their children (FP64 accumulation).

@@ -64,3 +64,39 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(nbx, "LIB_PATH", str(tmp_path / "libnbx.so"))
     with pytest.raises(RuntimeError, match="not built"):
         nbx.lib()
+
+
+def test_next_interface_mirror_compiles_and_runs(tmp_path):
+    """include/nbx_next.hpp offers NEXT's computeAdaptiveDt/Step signatures
+    for a struct with NEXT's Particle fields; built and run here it reports
+    the missing GPU as nbx_next::Error (NBX_ERR_NODEVICE), on a GPU box it
+    steps."""
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text(r'''
+#include "nbx_next.hpp"
+#include <cstdio>
+#include <vector>
+struct Particle { std::vector<double> x, y, z, vx, vy, vz, ax, ay, az, m; std::vector<int> type; };
+using ParticleSystem = Particle;
+int main() {
+  ParticleSystem p;
+  for (int i = 0; i < 3; ++i) {
+    p.x.push_back(0.1 * i); p.y.push_back(0.2 * i * i); p.z.push_back(-0.1 * i);
+    p.vx.push_back(0); p.vy.push_back(0); p.vz.push_back(0); p.m.push_back(1.0 / 3); p.type.push_back(0);
+  }
+  try {
+    double dt = nbx_next::computeAdaptiveDt(p, 1e-3);
+    nbx_next::Step(p, dt);
+    std::printf("stepped %.17g\n", p.x[1]);
+  } catch (const nbx_next::Error& e) {
+    std::printf("error %d\n", (int)e.status);
+  }
+}
+''')
+    exe = tmp_path / "use"
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L",
+                    os.path.join(ROOT, "next_amd"), "-lnbx", f"-Wl,-rpath,{os.path.join(ROOT, 'next_amd')}",
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120).stdout
+    assert out.startswith("stepped") or out.strip() == "error -6"
