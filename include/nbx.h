@@ -115,15 +115,39 @@ nbx_status nbx_download(nbx_ctx* ctx, double* x, double* y, double* z, double* v
 
 int64_t nbx_size(const nbx_ctx* ctx);
 
+/* ---- several independent systems in one context (ensembles, serving) ----
+ * No counterpart in NEXT, which runs one system per process.  Systems never
+ * interact; every system's trajectory is the same as in a context of its
+ * own.  nbx_step / nbx_run / nbx_accel / nbx_download act on all of them. */
+
+/* Copy nsys systems in: counts[s] bodies each (> 0), arrays concatenated in
+ * system order.  Replaces any previous contents. */
+nbx_status nbx_upload_batch(nbx_ctx* ctx, int32_t nsys, const int64_t* counts, const double* x, const double* y,
+                            const double* z, const double* vx, const double* vy, const double* vz,
+                            const double* m, const int32_t* type);
+/* Append one system of n bodies; *sys_out (may be null) receives its index. */
+nbx_status nbx_add_system(nbx_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                          const double* vx, const double* vy, const double* vz, const double* m,
+                          const int32_t* type, int32_t* sys_out);
+/* Copy system s out (its own upload order); nbx_remove_system also removes
+ * it, and the systems after it move down by one index. */
+nbx_status nbx_download_system(nbx_ctx* ctx, int32_t s, double* x, double* y, double* z, double* vx, double* vy,
+                               double* vz);
+nbx_status nbx_remove_system(nbx_ctx* ctx, int32_t s, double* x, double* y, double* z, double* vx, double* vy,
+                             double* vz);
+int32_t nbx_num_systems(const nbx_ctx* ctx);
+int64_t nbx_system_size(const nbx_ctx* ctx, int32_t s);
+
 /* = computeAdaptiveDt (adaptive.h:23-52) on the device state. */
 nbx_status nbx_adaptive_dt(nbx_ctx* ctx, double base_dt, double* dt_out);
 
-/* = Step (step.h:204-383): one kick-drift-kick step of size dt. */
+/* = Step (step.h:204-383): one kick-drift-kick step of size dt (every system). */
 nbx_status nbx_step(nbx_ctx* ctx, double dt);
 
 /* nsteps iterations of the driver loop body (begrun.cpp:97-99): dt from
- * the current speeds, then one step, without host round trips.  dts_out
- * (may be null) receives the nsteps time steps. */
+ * the current speeds (each system its own), then one step, without host
+ * round trips; with cfg.adaptive_dt = 0 every step is base_dt.  dts_out (may
+ * be null) receives the nsteps time steps of system 0. */
 nbx_status nbx_run(nbx_ctx* ctx, double base_dt, int32_t nsteps, double* dts_out);
 
 /* Accelerations at the current positions (bhAccel over all targets,
@@ -142,6 +166,8 @@ nbx_status nbx_direct(nbx_ctx* ctx, double* ax, double* ay, double* az, double* 
  * order: out5 = {kinetic, potential, px, py, pz}.  source 0: potential from
  * the tree walk (NBX_MODE_PHYSICS), 1: from the FP64 direct sum. */
 nbx_status nbx_energy(nbx_ctx* ctx, int source, double* out5);
+/* The same per system: out receives 5 values per system. */
+nbx_status nbx_energy_systems(nbx_ctx* ctx, int source, double* out);
 
 /* Accumulated timings and counters. */
 nbx_status nbx_get_timing(nbx_ctx* ctx, nbx_timing* out);
@@ -167,6 +193,53 @@ nbx_status nbx_debug_tree_size(nbx_ctx* ctx, int64_t* nodes);
 nbx_status nbx_debug_tree(nbx_ctx* ctx, int32_t* depth, int32_t* leaf, int32_t* body,
                           int32_t* cmask, double* geo4, double* mass, double* com3, double* q6,
                           int32_t* perm);
+
+/* ---- serving: a continuous-batching request loop -----------------------
+ * No counterpart in NEXT.  Each request is an independent simulation with
+ * its own settings.  Requests are admitted first come, first served while
+ * their estimated device memory fits the budget; admitted requests with the
+ * same settings share a batch context and step together, joining and leaving
+ * between steps; batches with different settings overlap on the device.  A
+ * request's result does not depend on the others. */
+
+typedef struct nbx_server nbx_server;
+
+typedef struct nbx_request {
+  int64_t n;
+  const double *x, *y, *z, *vx, *vy, *vz, *m; /* copied at submission */
+  const int32_t* type;
+  int32_t nsteps;     /* KDK steps to make */
+  double dt;          /* step (base step if adaptive_dt) */
+  int mode;           /* nbx_mode */
+  int precision;      /* nbx_precision */
+  double theta;
+  double eps;
+  int leaf_size;
+  int adaptive_dt;
+} nbx_request;
+
+typedef struct nbx_request_timing {
+  double queue_ms;    /* submission to first step */
+  double latency_ms;  /* submission to completion */
+} nbx_request_timing;
+
+typedef struct nbx_server_info {
+  int64_t queued, running, batches;
+  uint64_t bytes_in_use;
+  int64_t body_steps;   /* bodies x steps completed */
+  int64_t batch_steps;  /* batched steps launched */
+} nbx_server_info;
+
+nbx_status nbx_server_create(nbx_server** out, int device, uint64_t budget_bytes);
+void nbx_server_destroy(nbx_server* srv);
+nbx_status nbx_server_submit(nbx_server* srv, const nbx_request* req, int64_t* id);
+/* 0 queued, 1 running, 2 done, 3 failed; negative: unknown id. */
+int nbx_server_poll(nbx_server* srv, int64_t id);
+/* Block until the request is done, copy its final positions and velocities
+ * (any pointer may be null) and release it. */
+nbx_status nbx_server_wait(nbx_server* srv, int64_t id, double* x, double* y, double* z, double* vx, double* vy,
+                           double* vz, nbx_request_timing* timing);
+nbx_status nbx_server_stats(nbx_server* srv, nbx_server_info* out);
 
 #ifdef __cplusplus
 }

@@ -30,13 +30,9 @@ struct alignas(128) RefNode {
 };
 static_assert(sizeof(RefNode) == 128, "RefNode layout");
 
-// Small per-context device block: geometry, time step, flags and the
-// per-level node counts the host needs to launch the upward pass.
+// Small per-context device block: flags and the per-level node counts the
+// host needs to launch the upward pass.
 struct DevScalars {
-  double root[4];           // centre x, y, z, half-width
-  double dt;                // current time step
-  double base_dt;
-  unsigned long long v2max; // bit pattern of max |v|^2 (non-negative doubles order as integers)
   int32_t err;              // bit 0: non-finite position
   int32_t collisions;       // adjacent equal primary keys
   int32_t coincident;       // adjacent equal primary and secondary keys
@@ -46,6 +42,28 @@ struct DevScalars {
   int32_t level_count[kMaxDepthSlots];  // internal nodes per depth
   unsigned long long n_pp, n_pc;        // interaction counters
   unsigned long long n_open, n_iter;    // opened nodes, wave iterations
+};
+
+// Per-system arrays: a context holds one or more independent particle
+// systems, stored one after another in key order.
+struct SysArrays {
+  double* root;                 // 4 per system: centre x, y, z, half-width
+  double* dt;                   // time step of the current step
+  unsigned long long* v2max;    // bit pattern of max |v|^2 (non-negative doubles order as integers)
+  int32_t* soff;                // first body of each system; soff[nsys] = n
+  int32_t* sroot;               // preorder index of each system's root
+  int32_t* send;                // preorder index after each system's tree
+  int32_t nsys;
+};
+
+// One wave of walk targets: up to 64 consecutive bodies of one system.
+struct Group {
+  int32_t first, count, sys, pad;
+};
+
+// A block of the segmented bounding-box reduction: bodies [begin, end) of one system.
+struct BoxBlock {
+  int32_t sys, begin, end, pad;
 };
 
 // ---- wave primitives ------------------------------------------------------

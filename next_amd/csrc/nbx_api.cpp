@@ -1,16 +1,21 @@
 // C ABI of the engine (include/nbx.h): context, device buffers and the
 // host-side orchestration of one KDK step on one HIP stream.
 //
-// One step (step.h:204-383, with the second tree of step n reused as the
-// first tree of step n+1: the walk is deterministic, so a(x_{n+1}) from the
-// closing kick of step n is bit-identical to the opening-kick force of
-// step n+1):
+// A context holds one or more independent particle systems, stored one after
+// another.  One step of all of them (step.h:204-383, with the second tree of
+// step n reused as the first tree of step n+1: the walk is deterministic, so
+// a(x_{n+1}) from the closing kick of step n is bit-identical to the
+// opening-kick force of step n+1):
 //
-//   kick(a_n, dt/2) + drift(dt) + bbox partials     one fused kernel
-//   root, keys, radix sort, state gather            tree of x_{n+1}
+//   kick(a_n, dt/2) + drift(dt)                    one kernel
+//   per-system root cube, keys, radix sort by key and then by system,
+//   state gather                                   tree of x_{n+1}
 //   lcp/counts, scans  -> host reads node counts (one small copy)
 //   node emit, depth sorts, skip pairing, upward pass per depth
-//   walk + closing kick(a_{n+1}, dt/2) + max |v|^2  one fused kernel
+//   walk over the group table + closing kick(a_{n+1}, dt/2) + max |v|^2
+//
+// Systems never share a tree node or a walk group, so each system's result
+// is the same alone and in a batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,8 +26,8 @@
 
 #include "common.hpp"
 #include "nbx.h"
-#include "prims.hpp"
 #include "phys_kernels.hpp"
+#include "prims.hpp"
 #include "ref_kernels.hpp"
 
 using namespace nbx;
@@ -35,6 +40,7 @@ struct DBuf {
 };
 
 enum { EV_START, EV_KICK, EV_KEYS, EV_SORT, EV_BUILD, EV_UPWARD, EV_WALK0, EV_WALK, EV_COUNT };
+constexpr int kStateArrays = 10;   // x y z vx vy vz m type id sys
 
 }  // namespace
 
@@ -43,16 +49,23 @@ struct nbx_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int64_t n = 0;
-  // state, double-buffered for the gather into key order
-  DBuf st[2][9];
+  int64_t cap = 0;                 // particle capacity of the state buffers
+  // state in key order, double-buffered for the gather after each sort
+  DBuf st[2][kStateArrays];
   int cur = 0;
   DBuf acc[3];
   DBuf key[2], idx[2], key2[2];
   DBuf lcp, cnt_s, cnt_e;
   DBuf nodes;
   DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
-  DBuf scratch, bbox_part, dts, lvl_part;
-  DBuf pnodes, paux, pot, epart, eout;   // physics-mode tree, moments, potential, energy
+  DBuf scratch, dts, lvl_part;
+  DBuf pnodes, paux, pot, epart, eout;     // physics-mode tree, moments, potential, energy
+  // systems
+  std::vector<int64_t> sys_n;              // bodies per system
+  DBuf s_root, s_dt, s_v2, s_soff, s_sroot, s_send;
+  DBuf groups, boxes, box_off, box_part;
+  int64_t ngroups = 0;
+  int nboxes = 0;
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
   bool physics = false;       // NBX_MODE_PHYSICS
@@ -90,17 +103,23 @@ nbx_status fail(nbx_ctx* c, nbx_status s, const std::string& msg) {
     if (s_ != NBX_OK) return s_;              \
   } while (0)
 
-// Grow a buffer to at least `bytes` (contents not kept).
-nbx_status ensure(nbx_ctx* c, DBuf& b, size_t bytes) {
+// Grow a buffer to at least `bytes`; `keep` bytes of the old contents are
+// copied over (0: contents not kept).
+nbx_status ensure(nbx_ctx* c, DBuf& b, size_t bytes, size_t keep = 0) {
   if (bytes <= b.bytes && b.p) return NBX_OK;
-  if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
   const size_t want = std::max<size_t>(bytes, 256);
-  hipError_t e = hipMalloc(&b.p, want);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
   if (e != hipSuccess) {
-    b.p = nullptr;
     (void)hipGetLastError();
     return fail(c, NBX_ERR_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
   }
+  if (keep && b.p) {
+    HIPCHK(c, hipMemcpyAsync(p, b.p, std::min(keep, b.bytes), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (b.p) (void)hipFree(b.p);
+  b.p = p;
   b.bytes = want;
   return NBX_OK;
 }
@@ -111,10 +130,17 @@ void release(DBuf& b) {
   b.bytes = 0;
 }
 
+size_t elem_bytes(int k) { return k < 7 ? 8 : 4; }
+
 StateArrays view(nbx_ctx* c, int which) {
   DBuf* s = c->st[which];
   return StateArrays{P<double>(s[0]), P<double>(s[1]), P<double>(s[2]), P<double>(s[3]), P<double>(s[4]),
-                     P<double>(s[5]), P<double>(s[6]), P<int32_t>(s[7]), P<int32_t>(s[8])};
+                     P<double>(s[5]), P<double>(s[6]), P<int32_t>(s[7]), P<int32_t>(s[8]), P<int32_t>(s[9])};
+}
+
+SysArrays sysa(nbx_ctx* c) {
+  return SysArrays{P<double>(c->s_root), P<double>(c->s_dt), P<unsigned long long>(c->s_v2), P<int32_t>(c->s_soff),
+                   P<int32_t>(c->s_sroot), P<int32_t>(c->s_send), (int32_t)c->sys_n.size()};
 }
 
 void record(nbx_ctx* c, int e) {
@@ -127,48 +153,120 @@ nbx_status sync_scalars(nbx_ctx* c) {
   return NBX_OK;
 }
 
-// Buffers sized by the particle count.
-nbx_status ensure_particles(nbx_ctx* c, int64_t n) {
-  const size_t nn = (size_t)std::max<int64_t>(n, 1);
-  for (int w = 0; w < 2; ++w)
-    for (int k = 0; k < 9; ++k) NBXCHK(ensure(c, c->st[w][k], nn * (k < 7 ? 8 : 4)));
-  for (int k = 0; k < 3; ++k) NBXCHK(ensure(c, c->acc[k], nn * 8));
-  for (int w = 0; w < 2; ++w) {
-    NBXCHK(ensure(c, c->key[w], nn * 8));
-    NBXCHK(ensure(c, c->idx[w], nn * 4));
+// Buffers sized by the particle count; the current state (and the cached
+// accelerations) are kept when they grow.
+nbx_status ensure_particles(nbx_ctx* c, int64_t n, int64_t keep_n) {
+  const int64_t want = std::max<int64_t>(n, 1);
+  if (want > c->cap) {
+    const int64_t cap = std::max<int64_t>(want, c->cap + c->cap / 2);
+    for (int w = 0; w < 2; ++w)
+      for (int k = 0; k < kStateArrays; ++k)
+        NBXCHK(ensure(c, c->st[w][k], (size_t)cap * elem_bytes(k), w == c->cur ? (size_t)keep_n * elem_bytes(k) : 0));
+    for (int k = 0; k < 3; ++k) NBXCHK(ensure(c, c->acc[k], (size_t)cap * 8, (size_t)keep_n * 8));
+    for (int w = 0; w < 2; ++w) {
+      NBXCHK(ensure(c, c->key[w], (size_t)cap * 8));
+      NBXCHK(ensure(c, c->idx[w], (size_t)cap * 4));
+    }
+    NBXCHK(ensure(c, c->lcp, (size_t)cap * 4));
+    NBXCHK(ensure(c, c->cnt_s, (size_t)cap * 4));
+    NBXCHK(ensure(c, c->cnt_e, (size_t)cap * 4));
+    NBXCHK(ensure(c, c->lvl_part, (size_t)lcp_part_words(cap) * 4));
+    NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(cap), scan_scratch_words(cap)) * 4));
+    c->cap = cap;
   }
-  NBXCHK(ensure(c, c->lcp, nn * 4));
-  NBXCHK(ensure(c, c->cnt_s, nn * 4));
-  NBXCHK(ensure(c, c->cnt_e, nn * 4));
-  NBXCHK(ensure(c, c->bbox_part, (size_t)kBboxBlocks * 6 * 8));
-  NBXCHK(ensure(c, c->lvl_part, (size_t)lcp_part_words(std::max<int64_t>(n, 1)) * 4));
-  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(n), scan_scratch_words(n)) * 4));
   return NBX_OK;
 }
 
-// Tree of the current positions: keys, sort, gather, layout, multipoles.
-// `bbox_ready`: the root partials were produced by the fused kick/drift.
-nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
+// Host-built tables for the current systems: body offsets, walk groups (64
+// bodies of one system each), bounding-box blocks.
+nbx_status set_systems(nbx_ctx* c) {
+  const int S = (int)c->sys_n.size();
+  std::vector<int32_t> soff(S + 1, 0);
+  for (int s = 0; s < S; ++s) soff[s + 1] = soff[s] + (int32_t)c->sys_n[s];
+  std::vector<Group> groups;
+  std::vector<BoxBlock> boxes;
+  std::vector<int32_t> box_off(S + 1, 0);
+  for (int s = 0; s < S; ++s) {
+    for (int32_t b = soff[s]; b < soff[s + 1]; b += kWave)
+      groups.push_back(Group{b, std::min<int32_t>(kWave, soff[s + 1] - b), s, 0});
+    for (int32_t b = soff[s]; b < soff[s + 1]; b += kBoxSpan)
+      boxes.push_back(BoxBlock{s, b, std::min<int32_t>(b + kBoxSpan, soff[s + 1]), 0});
+    box_off[s + 1] = (int32_t)boxes.size();
+  }
+  const size_t SS = (size_t)std::max(S, 1);
+  NBXCHK(ensure(c, c->s_root, SS * 4 * 8));
+  NBXCHK(ensure(c, c->s_dt, SS * 8));
+  NBXCHK(ensure(c, c->s_v2, SS * 8));
+  NBXCHK(ensure(c, c->s_soff, (SS + 1) * 4));
+  NBXCHK(ensure(c, c->s_sroot, SS * 4));
+  NBXCHK(ensure(c, c->s_send, SS * 4));
+  NBXCHK(ensure(c, c->groups, std::max<size_t>(groups.size(), 1) * sizeof(Group)));
+  NBXCHK(ensure(c, c->boxes, std::max<size_t>(boxes.size(), 1) * sizeof(BoxBlock)));
+  NBXCHK(ensure(c, c->box_off, (SS + 1) * 4));
+  NBXCHK(ensure(c, c->box_part, std::max<size_t>(boxes.size(), 1) * 6 * 8));
+  NBXCHK(ensure(c, c->epart, std::max<size_t>(boxes.size(), 1) * 5 * 8));
+  NBXCHK(ensure(c, c->eout, SS * 5 * 8));
+  HIPCHK(c, hipMemcpyAsync(c->s_soff.p, soff.data(), soff.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if (!groups.empty())
+    HIPCHK(c, hipMemcpyAsync(c->groups.p, groups.data(), groups.size() * sizeof(Group), hipMemcpyHostToDevice,
+                             c->stream));
+  if (!boxes.empty())
+    HIPCHK(c, hipMemcpyAsync(c->boxes.p, boxes.data(), boxes.size() * sizeof(BoxBlock), hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->box_off.p, box_off.data(), box_off.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // the host vectors go out of scope
+  c->ngroups = (int64_t)groups.size();
+  c->nboxes = (int)boxes.size();
+  c->acc_valid = false;
+  c->tree_valid = false;
+  return NBX_OK;
+}
+
+int sys_bits(int nsys) {
+  int b = 0;
+  while ((1 << b) < nsys) ++b;
+  return b;
+}
+
+// Tree of the current positions: per-system roots, keys, sort, gather,
+// layout, multipoles.
+nbx_status build_tree(nbx_ctx* c) {
   const int64_t n = c->n;
+  const int S = (int)c->sys_n.size();
   hipStream_t s = c->stream;
-  StateArrays S = view(c, c->cur);
-  if (!bbox_ready) launch_bbox(n, S, P<double>(c->bbox_part), c->sc, s);
+  SysArrays sy = sysa(c);
+  StateArrays St = view(c, c->cur);
+  launch_bbox(St, P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), P<double>(c->box_part), sy, s);
   launch_fill_scalars(c->sc, 1 | 4, s);
-  launch_keys(n, S, c->sc, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
+  launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
   record(c, EV_KEYS);
   uint32_t* scr = P<uint32_t>(c->scratch);
   const bool in1 = radix_sort_u64(P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key[1]),
                                   P<uint32_t>(c->idx[1]), n, 63, scr, s);
   const uint64_t* skey = P<uint64_t>(c->key[in1 ? 1 : 0]);
   const uint32_t* perm = P<uint32_t>(c->idx[in1 ? 1 : 0]);
-  launch_gather_state(n, perm, S, view(c, 1 - c->cur), s);
+  if (S > 1) {
+    // stable pass by system: system-major, key order within each system
+    uint32_t* pa = P<uint32_t>(c->idx[in1 ? 1 : 0]);
+    uint32_t* pb = P<uint32_t>(c->idx[in1 ? 0 : 1]);
+    uint32_t* ka = P<uint32_t>(c->key[in1 ? 0 : 1]);       // the free key buffer, as 2n words
+    uint32_t* kb = ka + n;
+    launch_gather_sys(n, pa, St.sys, ka, s);
+    const bool r = radix_sort_u32(ka, pa, kb, pb, n, sys_bits(S), scr, s);
+    perm = r ? pb : pa;
+  }
+  launch_gather_state(n, perm, St, view(c, 1 - c->cur), s);
   c->cur = 1 - c->cur;
-  S = view(c, c->cur);
+  St = view(c, c->cur);
+  if (S > 1) {   // keys of the system-major order
+    launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
+    skey = P<uint64_t>(c->key[0]);
+  }
   record(c, EV_SORT);
 
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
   uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
-  launch_lcp_count(n, skey, nullptr, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
+  launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
                    P<uint32_t>(c->lvl_part), c->sc, s);
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
   scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
@@ -177,30 +275,38 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
 
   const uint64_t* skey2 = nullptr;
   if (c->hsc->collisions > 0) {
-    // Bodies sharing all 21 digits of the first key word: sort by
-    // (first word, second word).  The state is already in first-word order;
-    // a stable sort by the second word followed by a stable sort by the first
-    // gives the lexicographic order.
-    for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)n * 8));
-    const int fk = in1 ? 1 : 0;   // buffer holding the sorted first words
-    launch_keys2(n, S, c->sc, P<uint64_t>(c->key2[0]), s);
+    // Bodies of one system sharing all 21 digits of the first key word: sort
+    // by (system, first word, second word), least significant first.
+    for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)c->cap * 8));
+    launch_keys2(n, St, sy, P<uint64_t>(c->key2[0]), s);
     launch_iota(n, P<uint32_t>(c->idx[0]), s);
     const bool a = radix_sort_u64(P<uint64_t>(c->key2[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key2[1]),
                                   P<uint32_t>(c->idx[1]), n, 63, scr, s);
     uint32_t* p1 = P<uint32_t>(c->idx[a ? 1 : 0]);
     uint32_t* p1o = P<uint32_t>(c->idx[a ? 0 : 1]);
-    uint64_t* kfree = P<uint64_t>(c->key[1 - fk]);
-    launch_gather_u64(n, p1, P<uint64_t>(c->key[fk]), kfree, s);
-    const bool b = radix_sort_u64(kfree, p1, P<uint64_t>(c->key[fk]), p1o, n, 63, scr, s);
-    skey = b ? P<uint64_t>(c->key[fk]) : kfree;
+    // first words of the current order, recomputed from the positions
+    launch_keys(n, St, sy, P<uint64_t>(c->key[0]), p1o, c->sc, s);
+    launch_gather_u64(n, p1, P<uint64_t>(c->key[0]), P<uint64_t>(c->key[1]), s);
+    const bool b = radix_sort_u64(P<uint64_t>(c->key[1]), p1, P<uint64_t>(c->key[0]), p1o, n, 63, scr, s);
     const uint32_t* pf = b ? p1o : p1;
-    launch_gather_state(n, pf, S, view(c, 1 - c->cur), s);
+    if (S > 1) {
+      uint32_t* pa = b ? p1o : p1;
+      uint32_t* pb = b ? p1 : p1o;
+      uint32_t* ka = P<uint32_t>(c->key2[0]);
+      uint32_t* kb = ka + n;
+      launch_gather_sys(n, pa, St.sys, ka, s);
+      const bool r = radix_sort_u32(ka, pa, kb, pb, n, sys_bits(S), scr, s);
+      pf = r ? pb : pa;
+    }
+    launch_gather_state(n, pf, St, view(c, 1 - c->cur), s);
     c->cur = 1 - c->cur;
-    S = view(c, c->cur);
-    launch_keys2(n, S, c->sc, P<uint64_t>(c->key2[0]), s);
+    St = view(c, c->cur);
+    launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
+    launch_keys2(n, St, sy, P<uint64_t>(c->key2[0]), s);
+    skey = P<uint64_t>(c->key[0]);
     skey2 = P<uint64_t>(c->key2[0]);
     launch_fill_scalars(c->sc, 1, s);
-    launch_lcp_count(n, skey, skey2, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
+    launch_lcp_count(n, skey, skey2, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
                      P<uint32_t>(c->lvl_part), c->sc, s);
     scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
     scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
@@ -211,6 +317,7 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
   }
   c->skey = skey;
   c->skey2 = skey2;
+  launch_sys_roots(sy, P<uint32_t>(c->cnt_s), c->sc, s);
 
   const int64_t total = (uint32_t)c->hsc->total_nodes;
   const int64_t m = (uint32_t)c->hsc->internal_nodes;
@@ -221,10 +328,10 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
   for (DBuf* b : {&c->st_key, &c->st_val, &c->en_key, &c->en_val, &c->st_key2, &c->st_val2, &c->en_key2,
                   &c->en_val2, &c->level_list})
     NBXCHK(ensure(c, *b, mm));
-  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(std::max(n, m)), scan_scratch_words(n)) * 4));
+  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(std::max(c->cap, m)), scan_scratch_words(c->cap)) * 4));
   scr = P<uint32_t>(c->scratch);
   RefNode* nodes = P<RefNode>(c->nodes);
-  launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), S, c->sc, nodes,
+  launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, sy, c->sc, nodes,
               P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), s);
   const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
                                  P<uint32_t>(c->st_val2), m, 8, scr, s);
@@ -239,7 +346,7 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
   if (c->physics) {
     NBXCHK(ensure(c, c->pnodes, (size_t)total * sizeof(PhysNode)));
     NBXCHK(ensure(c, c->paux, (size_t)total * kPhysAuxBytes));
-    launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), S, P<PhysNode>(c->pnodes), c->paux.p, s);
+    launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), St, P<PhysNode>(c->pnodes), c->paux.p, s);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
         launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<PhysNode>(c->pnodes),
@@ -257,15 +364,15 @@ nbx_status build_tree(nbx_ctx* c, bool bbox_ready) {
 nbx_status walk(nbx_ctx* c, bool kick) {
   const bool count = (c->prof & 2) != 0;
   if (count) launch_fill_scalars(c->sc, 8, c->stream);
-  if (kick) launch_fill_scalars(c->sc, 2, c->stream);
+  if (kick) launch_fill_v2max(sysa(c), c->stream);
   record(c, EV_WALK0);
   if (c->physics)
-    launch_phys_walk(c->n, c->total_nodes, P<PhysNode>(c->pnodes), view(c, c->cur), c->cfg.eps, c->cfg.leaf_size,
-                     P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr, c->sc, kick, count,
-                     c->cfg.precision == NBX_PREC_FP64, c->stream);
+    launch_phys_walk(P<Group>(c->groups), c->ngroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
+                     c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr,
+                     c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64, c->stream);
   else
-    launch_walk(c->n, c->total_nodes, P<RefNode>(c->nodes), view(c, c->cur), c->cfg.theta, P<double>(c->acc[0]),
-                P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
+    launch_walk(P<Group>(c->groups), c->ngroups, P<RefNode>(c->nodes), view(c, c->cur), sysa(c), c->cfg.theta,
+                P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
   record(c, EV_WALK);
   HIPCHK(c, hipGetLastError());
   c->acc_valid = true;
@@ -291,8 +398,7 @@ nbx_status finish_timing(nbx_ctx* c, bool with_kick) {
     c->tim.total_ms += elapsed(c, EV_START, EV_WALK);
   }
   if (c->prof & 2) {
-    HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    NBXCHK(sync_scalars(c));
     c->tim.n_pp += (int64_t)c->hsc->n_pp;
     c->tim.n_pc += (int64_t)c->hsc->n_pc;
     c->tim.n_open += (int64_t)c->hsc->n_open;
@@ -308,25 +414,22 @@ nbx_status finish_timing(nbx_ctx* c, bool with_kick) {
 nbx_status ensure_accel(nbx_ctx* c) {
   if (c->acc_valid) return NBX_OK;
   record(c, EV_START);
-  NBXCHK(build_tree(c, false));
+  NBXCHK(build_tree(c));
   NBXCHK(walk(c, false));
   return NBX_OK;
 }
 
-// One KDK step with the dt held in the device scalars.
+// One KDK step of every system with the dt held on the device.
 nbx_status step_device(nbx_ctx* c) {
   if (c->n == 0) return NBX_OK;
   NBXCHK(ensure_accel(c));
   record(c, EV_START);
-  launch_kick_drift(c->n, view(c, c->cur), P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc,
-                    P<double>(c->bbox_part), c->stream);
-  launch_bbox_final(P<double>(c->bbox_part),
-                    (int)std::min<int64_t>(std::max<int64_t>(div_up(c->n, kBlock), 1), kBboxBlocks), c->sc,
+  launch_kick_drift(c->n, view(c, c->cur), P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), sysa(c),
                     c->stream);
   c->acc_valid = false;
   c->tree_valid = false;
   record(c, EV_KICK);
-  NBXCHK(build_tree(c, true));
+  NBXCHK(build_tree(c));
   NBXCHK(walk(c, true));
   return finish_timing(c, true);
 }
@@ -337,8 +440,53 @@ nbx_status check_ctx(nbx_ctx* c) {
   return NBX_OK;
 }
 
-void unpermute(const std::vector<int32_t>& id, const double* src, double* dst) {
-  for (size_t k = 0; k < id.size(); ++k) dst[id[k]] = src[k];
+int64_t sys_offset(const nbx_ctx* c, int s) {
+  int64_t o = 0;
+  for (int k = 0; k < s; ++k) o += c->sys_n[(size_t)k];
+  return o;
+}
+
+// Copy bodies [off, off + cnt) of a key-order device array out, each placed
+// at its upload index minus `base`.
+nbx_status download_range(nbx_ctx* c, const void* dev, size_t esz, int64_t off, int64_t cnt, int64_t base,
+                          const std::vector<int32_t>& id, void* out) {
+  std::vector<char> tmp((size_t)cnt * esz);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), static_cast<const char*>(dev) + off * esz, (size_t)cnt * esz,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  char* o = static_cast<char*>(out);
+  for (int64_t k = 0; k < cnt; ++k)
+    std::memcpy(o + ((int64_t)id[(size_t)k] - base) * esz, &tmp[(size_t)k * esz], esz);
+  return NBX_OK;
+}
+
+// Upload indices (`id`) of the bodies in slots [off, off + cnt).
+nbx_status read_ids(nbx_ctx* c, int64_t off, int64_t cnt, std::vector<int32_t>& id) {
+  id.resize((size_t)cnt);
+  HIPCHK(c, hipMemcpyAsync(id.data(), P<int32_t>(c->st[c->cur][8]) + off, (size_t)cnt * 4, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBX_OK;
+}
+
+// Write n bodies of system `sys` into slots [at, at + n); upload indices
+// id0, id0 + 1, ...
+nbx_status put_bodies(nbx_ctx* c, int64_t at, int64_t n, int32_t sys, int64_t id0, const double* x, const double* y,
+                      const double* z, const double* vx, const double* vy, const double* vz, const double* m,
+                      const int32_t* type) {
+  if (n == 0) return NBX_OK;
+  const void* src[8] = {x, y, z, vx, vy, vz, m, type};
+  for (int k = 0; k < 8; ++k)
+    HIPCHK(c, hipMemcpyAsync(static_cast<char*>(c->st[c->cur][k].p) + at * elem_bytes(k), src[k],
+                             (size_t)n * elem_bytes(k), hipMemcpyHostToDevice, c->stream));
+  std::vector<int32_t> id((size_t)n), sv((size_t)n, sys);
+  for (int64_t i = 0; i < n; ++i) id[(size_t)i] = (int32_t)(id0 + i);
+  HIPCHK(c, hipMemcpyAsync(P<int32_t>(c->st[c->cur][8]) + at, id.data(), (size_t)n * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(P<int32_t>(c->st[c->cur][9]) + at, sv.data(), (size_t)n * 4, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBX_OK;
 }
 
 }  // namespace
@@ -392,7 +540,7 @@ nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
     if (hipEventCreate(&e) != hipSuccess) { nbx_destroy(c); return NBX_ERR_HIP; }
   std::memset(c->hsc, 0, sizeof(DevScalars));
   if (hipMemcpyAsync(c->sc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess) {
+      hipStreamSynchronize(c->stream) != hipSuccess || set_systems(c) != NBX_OK) {
     nbx_destroy(c);
     return NBX_ERR_HIP;
   }
@@ -410,8 +558,9 @@ void nbx_destroy(nbx_ctx* c) {
   for (auto& b : c->idx) release(b);
   for (auto& b : c->key2) release(b);
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
-                  &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->bbox_part,
-                  &c->dts, &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout})
+                  &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
+                  &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout, &c->s_root, &c->s_dt, &c->s_v2,
+                  &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->sc) (void)hipFree(c->sc);
@@ -432,26 +581,102 @@ nbx_status nbx_set_theta(nbx_ctx* c, double theta) {
 }
 
 int64_t nbx_size(const nbx_ctx* c) { return c ? c->n : -1; }
+int32_t nbx_num_systems(const nbx_ctx* c) { return c ? (int32_t)c->sys_n.size() : -1; }
+int64_t nbx_system_size(const nbx_ctx* c, int32_t s) {
+  return (c && s >= 0 && s < (int32_t)c->sys_n.size()) ? c->sys_n[(size_t)s] : -1;
+}
+
+nbx_status nbx_upload_batch(nbx_ctx* c, int32_t nsys, const int64_t* counts, const double* x, const double* y,
+                            const double* z, const double* vx, const double* vy, const double* vz, const double* m,
+                            const int32_t* type) {
+  NBXCHK(check_ctx(c));
+  if (nsys < 0 || (nsys > 0 && !counts)) return fail(c, NBX_ERR_ARG, "bad system count");
+  int64_t n = 0;
+  for (int s = 0; s < nsys; ++s) {
+    if (counts[s] <= 0) return fail(c, NBX_ERR_ARG, "every system needs at least one body");
+    n += counts[s];
+  }
+  if (n > (int64_t)INT32_MAX / 64) return fail(c, NBX_ERR_ARG, "particle count out of range");
+  if (n > 0 && !(x && y && z && vx && vy && vz && m && type)) return fail(c, NBX_ERR_ARG, "null particle array");
+  c->cur = 0;
+  NBXCHK(ensure_particles(c, n, 0));
+  c->n = n;
+  c->sys_n.assign(counts, counts + nsys);
+  int64_t at = 0;
+  for (int s = 0; s < nsys; ++s) {
+    NBXCHK(put_bodies(c, at, counts[s], s, at, x + at, y + at, z + at, vx + at, vy + at, vz + at, m + at, type + at));
+    at += counts[s];
+  }
+  return set_systems(c);
+}
 
 nbx_status nbx_upload(nbx_ctx* c, int64_t n, const double* x, const double* y, const double* z, const double* vx,
                       const double* vy, const double* vz, const double* m, const int32_t* type) {
+  if (!c) return NBX_ERR_ARG;
+  if (n < 0) return fail(c, NBX_ERR_ARG, "particle count out of range");
+  if (n == 0) return nbx_upload_batch(c, 0, nullptr, x, y, z, vx, vy, vz, m, type);
+  return nbx_upload_batch(c, 1, &n, x, y, z, vx, vy, vz, m, type);
+}
+
+nbx_status nbx_add_system(nbx_ctx* c, int64_t n, const double* x, const double* y, const double* z, const double* vx,
+                          const double* vy, const double* vz, const double* m, const int32_t* type, int32_t* sys_out) {
   NBXCHK(check_ctx(c));
-  if (n < 0 || n > (int64_t)INT32_MAX / 64) return fail(c, NBX_ERR_ARG, "particle count out of range");
-  if (n > 0 && !(x && y && z && vx && vy && vz && m && type)) return fail(c, NBX_ERR_ARG, "null particle array");
-  NBXCHK(ensure_particles(c, n));
-  c->n = n;
-  c->cur = 0;
-  c->acc_valid = false;
-  c->tree_valid = false;
-  if (n > 0) {
-    const void* src[8] = {x, y, z, vx, vy, vz, m, type};
-    for (int k = 0; k < 8; ++k)
-      HIPCHK(c, hipMemcpyAsync(c->st[0][k].p, src[k], (size_t)n * (k < 7 ? 8 : 4), hipMemcpyHostToDevice, c->stream));
-    std::vector<int32_t> id((size_t)n);
-    for (int64_t i = 0; i < n; ++i) id[(size_t)i] = (int32_t)i;
-    HIPCHK(c, hipMemcpyAsync(c->st[0][8].p, id.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n <= 0 || c->n + n > (int64_t)INT32_MAX / 64) return fail(c, NBX_ERR_ARG, "bad particle count");
+  if (!(x && y && z && vx && vy && vz && m && type)) return fail(c, NBX_ERR_ARG, "null particle array");
+  NBXCHK(ensure_particles(c, c->n + n, c->n));
+  const int32_t s = (int32_t)c->sys_n.size();
+  NBXCHK(put_bodies(c, c->n, n, s, c->n, x, y, z, vx, vy, vz, m, type));
+  c->n += n;
+  c->sys_n.push_back(n);
+  if (sys_out) *sys_out = s;
+  return set_systems(c);
+}
+
+nbx_status nbx_download_system(nbx_ctx* c, int32_t s, double* x, double* y, double* z, double* vx, double* vy,
+                               double* vz) {
+  NBXCHK(check_ctx(c));
+  if (s < 0 || s >= (int32_t)c->sys_n.size()) return fail(c, NBX_ERR_ARG, "no such system");
+  const int64_t off = sys_offset(c, s), cnt = c->sys_n[(size_t)s];
+  std::vector<int32_t> id;
+  NBXCHK(read_ids(c, off, cnt, id));
+  double* outs[6] = {x, y, z, vx, vy, vz};
+  for (int k = 0; k < 6; ++k)
+    if (outs[k]) NBXCHK(download_range(c, c->st[c->cur][k].p, 8, off, cnt, off, id, outs[k]));
+  return NBX_OK;
+}
+
+nbx_status nbx_remove_system(nbx_ctx* c, int32_t s, double* x, double* y, double* z, double* vx, double* vy,
+                             double* vz) {
+  NBXCHK(nbx_download_system(c, s, x, y, z, vx, vy, vz));
+  const int64_t off = sys_offset(c, s), cnt = c->sys_n[(size_t)s];
+  const int64_t n_new = c->n - cnt;
+  hipStream_t st = c->stream;
+  uint32_t* perm = P<uint32_t>(c->idx[0]);
+  launch_remove_map(n_new, off, cnt, perm, st);
+  launch_gather_state(n_new, perm, view(c, c->cur), view(c, 1 - c->cur), st);
+  c->cur = 1 - c->cur;
+  // the other systems keep their cached forces
+  for (int k = 0; k < 3; ++k) {
+    launch_gather_f64(n_new, perm, P<double>(c->acc[k]), P<double>(c->key[1]), st);
+    if (n_new > 0)
+      HIPCHK(c, hipMemcpyAsync(c->acc[k].p, c->key[1].p, (size_t)n_new * 8, hipMemcpyDeviceToDevice, st));
   }
+  launch_relabel(n_new, P<int32_t>(c->st[c->cur][9]), s, st);
+  // upload indices of later systems move down by cnt
+  std::vector<int32_t> ids((size_t)n_new);
+  if (n_new > 0) {
+    HIPCHK(c, hipMemcpyAsync(ids.data(), c->st[c->cur][8].p, (size_t)n_new * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    for (auto& v : ids)
+      if (v >= off + cnt) v -= (int32_t)cnt;
+    HIPCHK(c, hipMemcpyAsync(c->st[c->cur][8].p, ids.data(), (size_t)n_new * 4, hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(c, hipStreamSynchronize(st));
+  const bool kept = c->acc_valid;
+  c->n = n_new;
+  c->sys_n.erase(c->sys_n.begin() + s);
+  NBXCHK(set_systems(c));
+  c->acc_valid = kept && n_new > 0;   // the tree is rebuilt on the next step; the others' forces are unchanged
   return NBX_OK;
 }
 
@@ -459,34 +684,31 @@ nbx_status nbx_download(nbx_ctx* c, double* x, double* y, double* z, double* vx,
   NBXCHK(check_ctx(c));
   const int64_t n = c->n;
   if (n == 0) return NBX_OK;
-  std::vector<int32_t> id((size_t)n);
-  std::vector<double> tmp((size_t)n);
-  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur][8].p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int32_t> id;
+  NBXCHK(read_ids(c, 0, n, id));
   double* outs[6] = {x, y, z, vx, vy, vz};
-  for (int k = 0; k < 6; ++k) {
-    if (!outs[k]) continue;
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->st[c->cur][k].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    unpermute(id, tmp.data(), outs[k]);
-  }
+  for (int k = 0; k < 6; ++k)
+    if (outs[k]) NBXCHK(download_range(c, c->st[c->cur][k].p, 8, 0, n, 0, id, outs[k]));
   return NBX_OK;
 }
 
 nbx_status nbx_adaptive_dt(nbx_ctx* c, double base_dt, double* dt_out) {
   NBXCHK(check_ctx(c));
   if (!dt_out) return fail(c, NBX_ERR_ARG, "null dt_out");
-  launch_fill_scalars(c->sc, 2, c->stream);
-  if (c->n > 0) launch_maxv2(c->n, view(c, c->cur), c->sc, c->stream);
-  launch_dt(c->sc, base_dt, nullptr, 0, c->stream);
-  NBXCHK(sync_scalars(c));
-  *dt_out = c->hsc->dt;
+  if (c->n == 0) { *dt_out = base_dt; return NBX_OK; }   // computeAdaptiveDt of no particles
+  SysArrays sy = sysa(c);
+  launch_fill_v2max(sy, c->stream);
+  launch_maxv2(c->n, view(c, c->cur), sy, c->stream);
+  launch_dt(sy, base_dt, nullptr, 0, c->stream);
+  HIPCHK(c, hipMemcpyAsync(dt_out, c->s_dt.p, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBX_OK;
 }
 
 nbx_status nbx_step(nbx_ctx* c, double dt) {
   NBXCHK(check_ctx(c));
-  launch_set_dt(c->sc, dt, c->stream);
+  if (c->n == 0) return NBX_OK;
+  launch_set_dt(sysa(c), dt, c->stream);
   NBXCHK(step_device(c));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBX_OK;
@@ -496,23 +718,23 @@ nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) 
   NBXCHK(check_ctx(c));
   if (nsteps < 0) return fail(c, NBX_ERR_ARG, "negative step count");
   if (nsteps == 0) return NBX_OK;
-  if (c->n == 0) {  // computeAdaptiveDt of no particles is base_dt (and so is a fixed dt); Step is a no-op
+  if (c->n == 0 || !c->cfg.adaptive_dt) {   // no bodies: computeAdaptiveDt gives base_dt; fixed dt: base_dt
     if (dts_out) for (int s = 0; s < nsteps; ++s) dts_out[s] = base_dt;
-    return NBX_OK;
+    if (c->n == 0) return NBX_OK;
   }
   NBXCHK(ensure(c, c->dts, (size_t)nsteps * 8));
-  launch_fill_scalars(c->sc, 2, c->stream);
-  launch_maxv2(c->n, view(c, c->cur), c->sc, c->stream);
-  if (!c->cfg.adaptive_dt) launch_set_dt(c->sc, base_dt, c->stream);
+  SysArrays sy = sysa(c);
+  if (c->cfg.adaptive_dt) {
+    launch_fill_v2max(sy, c->stream);
+    launch_maxv2(c->n, view(c, c->cur), sy, c->stream);
+  } else {
+    launch_set_dt(sy, base_dt, c->stream);
+  }
   for (int32_t s = 0; s < nsteps; ++s) {
-    if (c->cfg.adaptive_dt) launch_dt(c->sc, base_dt, P<double>(c->dts), s, c->stream);
+    if (c->cfg.adaptive_dt) launch_dt(sy, base_dt, P<double>(c->dts), s, c->stream);
     NBXCHK(step_device(c));
   }
-  if (!c->cfg.adaptive_dt && dts_out) {
-    for (int s = 0; s < nsteps; ++s) dts_out[s] = base_dt;
-    dts_out = nullptr;
-  }
-  if (dts_out)
+  if (dts_out && c->cfg.adaptive_dt)
     HIPCHK(c, hipMemcpyAsync(dts_out, c->dts.p, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBX_OK;
@@ -525,42 +747,29 @@ nbx_status nbx_accel(nbx_ctx* c, double* ax, double* ay, double* az) {
   const bool fresh = !c->acc_valid;
   NBXCHK(ensure_accel(c));
   if (fresh) NBXCHK(finish_timing(c, false));
-  const int64_t n = c->n;
-  std::vector<int32_t> id((size_t)n);
-  std::vector<double> tmp((size_t)n);
-  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur][8].p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int32_t> id;
+  NBXCHK(read_ids(c, 0, c->n, id));
   double* outs[3] = {ax, ay, az};
-  for (int k = 0; k < 3; ++k) {
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), c->acc[k].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    unpermute(id, tmp.data(), outs[k]);
-  }
+  for (int k = 0; k < 3; ++k) NBXCHK(download_range(c, c->acc[k].p, 8, 0, c->n, 0, id, outs[k]));
   return NBX_OK;
 }
 
-// Copy a sorted-order device array out in upload order.
 static nbx_status download_unpermuted(nbx_ctx* c, const void* dev, double* out) {
-  const int64_t n = c->n;
-  std::vector<int32_t> id((size_t)n);
-  std::vector<double> tmp((size_t)n);
-  HIPCHK(c, hipMemcpyAsync(id.data(), c->st[c->cur][8].p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(tmp.data(), dev, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  unpermute(id, tmp.data(), out);
-  return NBX_OK;
+  std::vector<int32_t> id;
+  NBXCHK(read_ids(c, 0, c->n, id));
+  return download_range(c, dev, 8, 0, c->n, 0, id, out);
 }
 
 // Tree potential of the current positions (physics mode).
 static nbx_status tree_potential(nbx_ctx* c) {
   NBXCHK(ensure(c, c->pot, (size_t)std::max<int64_t>(c->n, 1) * 8));
   if (!c->tree_valid) {
-    NBXCHK(build_tree(c, false));
+    NBXCHK(build_tree(c));
     c->acc_valid = false;
   }
-  launch_phys_walk(c->n, c->total_nodes, P<PhysNode>(c->pnodes), view(c, c->cur), c->cfg.eps, c->cfg.leaf_size,
-                   P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), P<double>(c->pot), c->sc, false,
-                   false, c->cfg.precision == NBX_PREC_FP64, c->stream);
+  launch_phys_walk(P<Group>(c->groups), c->ngroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
+                   c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]),
+                   P<double>(c->pot), c->sc, false, false, c->cfg.precision == NBX_PREC_FP64, c->stream);
   HIPCHK(c, hipGetLastError());
   c->acc_valid = true;
   return NBX_OK;
@@ -582,7 +791,7 @@ nbx_status nbx_direct(nbx_ctx* c, double* ax, double* ay, double* az, double* po
   DBuf tmp;
   NBXCHK(ensure(c, tmp, (size_t)c->n * 8 * 4));
   double* d = P<double>(tmp);
-  launch_direct64(c->n, view(c, c->cur), c->cfg.eps, d, d + c->n, d + 2 * c->n, d + 3 * c->n, c->stream);
+  launch_direct64(c->n, view(c, c->cur), sysa(c), c->cfg.eps, d, d + c->n, d + 2 * c->n, d + 3 * c->n, c->stream);
   nbx_status st = NBX_OK;
   double* outs[4] = {ax, ay, az, pot};
   for (int k = 0; k < 4 && st == NBX_OK; ++k)
@@ -592,25 +801,39 @@ nbx_status nbx_direct(nbx_ctx* c, double* ax, double* ay, double* az, double* po
   return st;
 }
 
+nbx_status nbx_energy_systems(nbx_ctx* c, int source, double* out) {
+  NBXCHK(check_ctx(c));
+  if (!out) return fail(c, NBX_ERR_ARG, "null output");
+  if (source != 0 && source != 1) return fail(c, NBX_ERR_ARG, "source must be 0 (tree) or 1 (direct)");
+  if (source == 0 && !c->physics) return fail(c, NBX_ERR_ARG, "the tree potential exists in physics mode only");
+  const int S = (int)c->sys_n.size();
+  if (c->n == 0) return NBX_OK;
+  NBXCHK(ensure(c, c->pot, (size_t)c->n * 8));
+  if (source == 0) {
+    NBXCHK(tree_potential(c));
+  } else {
+    launch_direct64(c->n, view(c, c->cur), sysa(c), c->cfg.eps, P<double>(c->acc[0]), P<double>(c->acc[1]),
+                    P<double>(c->acc[2]), P<double>(c->pot), c->stream);
+    c->acc_valid = false;   // the direct-sum forces replaced the tree forces
+  }
+  launch_energy(view(c, c->cur), P<double>(c->pot), P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), S,
+                P<double>(c->epart), P<double>(c->eout), c->stream);
+  HIPCHK(c, hipMemcpyAsync(out, c->eout.p, (size_t)S * 5 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return NBX_OK;
+}
+
 nbx_status nbx_energy(nbx_ctx* c, int source, double* out5) {
   NBXCHK(check_ctx(c));
   if (!out5) return fail(c, NBX_ERR_ARG, "null output");
   if (source != 0 && source != 1) return fail(c, NBX_ERR_ARG, "source must be 0 (tree) or 1 (direct)");
-  if (source == 0 && !c->physics) return fail(c, NBX_ERR_ARG, "the tree potential exists in physics mode only");
-  if (c->n == 0) { for (int k = 0; k < 5; ++k) out5[k] = 0.0; return NBX_OK; }
-  NBXCHK(ensure(c, c->pot, (size_t)c->n * 8));
-  NBXCHK(ensure(c, c->epart, (size_t)kEnergyBlocks * 5 * 8));
-  NBXCHK(ensure(c, c->eout, 5 * 8));
-  if (source == 0) {
-    NBXCHK(tree_potential(c));
-  } else {
-    launch_direct64(c->n, view(c, c->cur), c->cfg.eps, P<double>(c->acc[0]), P<double>(c->acc[1]),
-                    P<double>(c->acc[2]), P<double>(c->pot), c->stream);
-    c->acc_valid = false;   // the direct-sum forces replaced the tree forces
-  }
-  launch_energy(c->n, view(c, c->cur), P<double>(c->pot), P<double>(c->epart), P<double>(c->eout), c->stream);
-  HIPCHK(c, hipMemcpyAsync(out5, c->eout.p, 5 * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 5; ++k) out5[k] = 0.0;
+  const int S = (int)c->sys_n.size();
+  if (c->n == 0) return NBX_OK;
+  std::vector<double> per((size_t)S * 5);
+  NBXCHK(nbx_energy_systems(c, source, per.data()));
+  for (int s = 0; s < S; ++s)
+    for (int k = 0; k < 5; ++k) out5[k] += per[(size_t)s * 5 + k];
   return NBX_OK;
 }
 
@@ -636,13 +859,15 @@ nbx_status nbx_debug_keys(nbx_ctx* c, double* root4, uint64_t* hi, uint64_t* lo)
   NBXCHK(check_ctx(c));
   const int64_t n = c->n;
   if (!root4) return fail(c, NBX_ERR_ARG, "null root4");
-  launch_bbox(n, view(c, c->cur), P<double>(c->bbox_part), c->sc, c->stream);
-  NBXCHK(sync_scalars(c));
-  std::memcpy(root4, c->hsc->root, 4 * sizeof(double));
   if (n == 0) return NBX_OK;
+  if (c->sys_n.size() != 1) return fail(c, NBX_ERR_ARG, "parity hooks need a single system");
   if (!(hi && lo)) return fail(c, NBX_ERR_ARG, "null key output");
-  for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)n * 8));
-  launch_keys_export(n, view(c, c->cur), c->sc, P<uint64_t>(c->key2[0]), P<uint64_t>(c->key2[1]), c->stream);
+  SysArrays sy = sysa(c);
+  launch_bbox(view(c, c->cur), P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), P<double>(c->box_part), sy,
+              c->stream);
+  for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)c->cap * 8));
+  launch_keys_export(n, view(c, c->cur), sy, P<uint64_t>(c->key2[0]), P<uint64_t>(c->key2[1]), c->stream);
+  HIPCHK(c, hipMemcpyAsync(root4, c->s_root.p, 32, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(hi, c->key2[0].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(lo, c->key2[1].p, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -655,8 +880,9 @@ nbx_status nbx_debug_tree_size(nbx_ctx* c, int64_t* nodes) {
   NBXCHK(check_ctx(c));
   if (!nodes) return fail(c, NBX_ERR_ARG, "null output");
   if (c->n == 0) { *nodes = 0; return NBX_OK; }
+  if (c->sys_n.size() != 1) return fail(c, NBX_ERR_ARG, "parity hooks need a single system");
   if (!c->tree_valid) {
-    NBXCHK(build_tree(c, false));
+    NBXCHK(build_tree(c));
     c->acc_valid = false;
   }
   *nodes = c->total_nodes;
@@ -683,8 +909,8 @@ nbx_status nbx_debug_tree(nbx_ctx* c, int32_t* depth, int32_t* leaf, int32_t* bo
   double* d_mass = d_geo + 4 * total;
   double* d_com = d_mass + total;
   double* d_q = d_com + 3 * total;
-  launch_export(total, P<RefNode>(c->nodes), c->skey, c->skey2, P<int32_t>(c->st[c->cur][8]), c->sc, d_depth, d_leaf,
-                d_body, d_cmask, d_geo, d_mass, d_com, d_q, c->stream);
+  launch_export(total, P<RefNode>(c->nodes), c->skey, c->skey2, P<int32_t>(c->st[c->cur][8]), sysa(c), d_depth,
+                d_leaf, d_body, d_cmask, d_geo, d_mass, d_com, d_q, c->stream);
   hipError_t e = hipMemcpyAsync(depth, d_depth, ni, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(leaf, d_leaf, ni, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(body, d_body, ni, hipMemcpyDeviceToHost, c->stream);

@@ -168,8 +168,9 @@ __device__ __forceinline__ int64_t xcd_group(int nb) {
 // lane order, so the lists, and every sum, come out in the same order on
 // every run.
 template <typename T, bool KICK, bool POT, bool COUNT>
-__global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t total, const PhysNode* __restrict__ pn,
-                                                          StateArrays st, double eps, int nleaf,
+__global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restrict__ groups, int64_t ngroups,
+                                                          const PhysNode* __restrict__ pn, StateArrays st,
+                                                          SysArrays sy, double eps, int nleaf,
                                                           double* __restrict__ ax, double* __restrict__ ay,
                                                           double* __restrict__ az, double* __restrict__ pot,
                                                           DevScalars* sc) {
@@ -183,13 +184,15 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t tot
   __shared__ V4<T> pl[kPhysParts];       // body: relative position, mass
 
   const int lane = threadIdx.x;
-  const int64_t g0 = xcd_group(gridDim.x) * kPhysBlock;
-  if (g0 >= n) return;                                     // whole wave past the end
-  const int64_t i = g0 + lane;
-  const bool valid = i < n;
-  const int64_t ii = valid ? i : g0;
+  const int64_t gi = xcd_group(gridDim.x);
+  if (gi >= ngroups) return;                               // whole wave past the end
+  const Group g = groups[gi];
+  const int64_t i = g.first + lane;
+  const bool valid = lane < g.count;
+  const int64_t ii = valid ? i : g.first;
   const double px = st.x[ii], py = st.y[ii], pz = st.z[ii];
   const uint64_t ltmask = lanemask_lt();
+  const int32_t root = sy.sroot[g.sys], total = sy.send[g.sys];   // this system's tree
 
   // group bounding box
   double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t tot
     np = 0;
   };
 
-  int32_t cur = lane == 0 ? 0 : total;                     // this lane's range [cur, end)
+  int32_t cur = lane == 0 ? root : total;                  // this lane's range [cur, end)
   int32_t end = total;
   int sp = 0;                                              // donated ranges on the stack
   for (;;) {
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t tot
     ax[i] = Ax; ay[i] = Ay; az[i] = Az;
     if (POT) pot[i] = P + st.m[i] / eps;                   // drop the self pair, -m_i/eps
     if (KICK) {
-      const double half = sc->dt * 0.5;
+      const double half = sy.dt[g.sys] * 0.5;
       const double vx = st.vx[i] + Ax * half;
       const double vy = st.vy[i] + Ay * half;
       const double vz = st.vz[i] + Az * half;
@@ -369,36 +372,44 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(int64_t n, int32_t tot
       const double o = __shfl_xor(v2, off, kWave);
       if (o > v2) v2 = o;
     }
-    if (lane == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
+    if (lane == 0 && v2 > 0.0) atomicMax(&sy.v2max[g.sys], (unsigned long long)__double_as_longlong(v2));
   }
   if (COUNT && lane == 0) {
-    const uint32_t nv = (uint32_t)(n - g0 < kPhysBlock ? n - g0 : kPhysBlock);
+    const uint32_t nv = (uint32_t)g.count;
     atomicAdd(&sc->n_pc, (unsigned long long)ncell_tot * nv);
     atomicAdd(&sc->n_pp, (unsigned long long)npart_tot * nv);
   }
 }
 
 // FP64 direct sum: each thread owns one target; source bodies are staged
-// through LDS in tiles of one block.
-__global__ __launch_bounds__(kBlock) void k_direct64(int64_t n, StateArrays st, double e2, double* __restrict__ ax,
-                                                     double* __restrict__ ay, double* __restrict__ az,
-                                                     double* __restrict__ pot) {
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sm[kBlock];
+// through LDS in tiles of one block.  Only bodies of the target's own system
+// act on it; the block loops over the union of its targets' systems.
+__global__ __launch_bounds__(kBlock) void k_direct64(int64_t n, StateArrays st, SysArrays sy, double e2,
+                                                     double* __restrict__ ax, double* __restrict__ ay,
+                                                     double* __restrict__ az, double* __restrict__ pot) {
+  __shared__ double sx[kBlock], sy_[kBlock], sz[kBlock], sm[kBlock];
+  __shared__ int32_t ss[kBlock];
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const bool valid = i < n;
+  const int64_t last = ((int64_t)blockIdx.x + 1) * kBlock - 1 < n ? ((int64_t)blockIdx.x + 1) * kBlock - 1 : n - 1;
+  const int32_t me = valid ? st.sys[i] : -1;
+  const int64_t lo = sy.soff[st.sys[(int64_t)blockIdx.x * kBlock]];
+  const int64_t hi = sy.soff[st.sys[last] + 1];
   const double px = valid ? st.x[i] : 0, py = valid ? st.y[i] : 0, pz = valid ? st.z[i] : 0;
   double a0 = 0, a1 = 0, a2 = 0, p = 0;
-  for (int64_t t = 0; t < n; t += kBlock) {
+  for (int64_t t = lo; t < hi; t += kBlock) {
     const int64_t j = t + threadIdx.x;
-    sx[threadIdx.x] = j < n ? st.x[j] : 0;
-    sy[threadIdx.x] = j < n ? st.y[j] : 0;
-    sz[threadIdx.x] = j < n ? st.z[j] : 0;
-    sm[threadIdx.x] = j < n ? st.m[j] : 0;
+    const bool in = j < hi;
+    sx[threadIdx.x] = in ? st.x[j] : 0;
+    sy_[threadIdx.x] = in ? st.y[j] : 0;
+    sz[threadIdx.x] = in ? st.z[j] : 0;
+    sm[threadIdx.x] = in ? st.m[j] : 0;
+    ss[threadIdx.x] = in ? st.sys[j] : -2;
     __syncthreads();
-    const int lim = (int)(n - t < kBlock ? n - t : kBlock);
+    const int lim = (int)(hi - t < kBlock ? hi - t : kBlock);
     for (int k = 0; k < lim; ++k) {
-      if (t + k == i) continue;
-      const double dx = sx[k] - px, dy = sy[k] - py, dz = sz[k] - pz;
+      if (t + k == i || ss[k] != me) continue;
+      const double dx = sx[k] - px, dy = sy_[k] - py, dz = sz[k] - pz;
       const double r2 = dx * dx + dy * dy + dz * dz + e2;
       const double ri = 1.0 / sqrt(r2);
       const double mr3 = sm[k] * ri * ri * ri;
@@ -413,11 +424,15 @@ __global__ __launch_bounds__(kBlock) void k_direct64(int64_t n, StateArrays st, 
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_energy_partial(int64_t n, StateArrays st, const double* __restrict__ pot,
+// Energy and momentum per system, over the segmented blocks of the
+// bounding-box reduction; each system's blocks are summed in block order.
+__global__ __launch_bounds__(kBlock) void k_energy_partial(StateArrays st, const double* __restrict__ pot,
+                                                           const BoxBlock* __restrict__ blocks,
                                                            double* __restrict__ part) {
   __shared__ double red[kBlock / kWave][5];
+  const BoxBlock b = blocks[blockIdx.x];
   double v[5] = {0, 0, 0, 0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+  for (int64_t i = b.begin + threadIdx.x; i < b.end; i += kBlock) {
     const double m = st.m[i], vx = st.vx[i], vy = st.vy[i], vz = st.vz[i];
     v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
     if (pot) v[1] += 0.5 * m * pot[i];
@@ -436,21 +451,24 @@ __global__ __launch_bounds__(kBlock) void k_energy_partial(int64_t n, StateArray
   }
 }
 
-__global__ void k_energy_final(const double* __restrict__ part, int nparts, double* out5) {
-  if (threadIdx.x < 5) {
-    double t = 0;
-    for (int b = 0; b < nparts; ++b) t += part[5 * b + threadIdx.x];
-    out5[threadIdx.x] = t;
-  }
+__global__ __launch_bounds__(kBlock) void k_energy_final(const double* __restrict__ part,
+                                                         const int32_t* __restrict__ block_off, int nsys,
+                                                         double* out) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= nsys) return;
+  double t[5] = {0, 0, 0, 0, 0};
+  for (int b = block_off[s]; b < block_off[s + 1]; ++b)
+    for (int k = 0; k < 5; ++k) t[k] += part[5 * b + k];
+  for (int k = 0; k < 5; ++k) out[5 * s + k] = t[k];
 }
 
 template <typename T>
-void walk_dispatch(unsigned nb, int64_t n, int32_t t, const PhysNode* pn, const StateArrays& st, double eps,
-                   int nleaf, double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count,
-                   hipStream_t s) {
-#define NBX_PW(K, P, C)                                                                                       \
-  hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, n, t, pn, st, eps, nleaf, ax, \
-                     ay, az, pot, sc)
+void walk_dispatch(unsigned nb, const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
+                   const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
+                   DevScalars* sc, bool kick, bool count, hipStream_t s) {
+#define NBX_PW(K, P, C)                                                                                     \
+  hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, groups, ngroups, pn, st, sy, \
+                     eps, nleaf, ax, ay, az, pot, sc)
   const bool p = pot != nullptr;
   if (kick) {
     if (count) NBX_PW(true, false, true); else NBX_PW(true, false, false);
@@ -478,26 +496,28 @@ void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, con
                        reinterpret_cast<PhysAux*>(aux), n, total, 1.0 / theta);
 }
 
-void launch_phys_walk(int64_t n, int64_t total, const PhysNode* pn, const StateArrays& st, double eps, int nleaf,
-                      double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count, bool fp64,
-                      hipStream_t s) {
-  if (n <= 0) return;
-  const unsigned nb = (unsigned)(div_up(div_up(n, kPhysBlock), 8) * 8);
+void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
+                      const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
+                      DevScalars* sc, bool kick, bool count, bool fp64, hipStream_t s) {
+  if (ngroups <= 0) return;
+  const unsigned nb = (unsigned)(div_up(ngroups, 8) * 8);
   if (fp64)
-    walk_dispatch<double>(nb, n, (int32_t)total, pn, st, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
+    walk_dispatch<double>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
   else
-    walk_dispatch<float>(nb, n, (int32_t)total, pn, st, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
+    walk_dispatch<float>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
 }
 
-void launch_direct64(int64_t n, const StateArrays& st, double eps, double* ax, double* ay, double* az, double* pot,
-                     hipStream_t s) {
+void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,
+                     double* az, double* pot, hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_direct64, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, st, eps * eps, ax, ay, az, pot);
+    hipLaunchKernelGGL(k_direct64, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, st, sy, eps * eps, ax, ay, az,
+                       pot);
 }
 
-void launch_energy(int64_t n, const StateArrays& st, const double* pot, double* part, double* out5, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_partial, dim3(kEnergyBlocks), dim3(kBlock), 0, s, n, st, pot, part);
-  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kWave), 0, s, part, kEnergyBlocks, out5);
+void launch_energy(const StateArrays& st, const double* pot, const BoxBlock* blocks, int nblocks,
+                   const int32_t* block_off, int nsys, double* part, double* out, hipStream_t s) {
+  if (nblocks > 0) hipLaunchKernelGGL(k_energy_partial, dim3(nblocks), dim3(kBlock), 0, s, st, pot, blocks, part);
+  hipLaunchKernelGGL(k_energy_final, dim3(grid_of(nsys, kBlock)), dim3(kBlock), 0, s, part, block_off, nsys, out);
 }
 
 }  // namespace nbx

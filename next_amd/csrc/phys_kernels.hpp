@@ -40,18 +40,20 @@ void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, 
 // Internal nodes of one depth from their children (FP64 accumulation).
 void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
                         void* aux, int64_t n, int64_t total, double theta, hipStream_t s);
-// Group walk: accelerations (and potentials if pot != null), optionally
-// followed by the closing half kick.  fp64 selects the FP64 evaluation.
-void launch_phys_walk(int64_t n, int64_t total, const PhysNode* pn, const StateArrays& st, double eps, int nleaf,
-                      double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick, bool count,
-                      bool fp64, hipStream_t s);
-// FP64 direct sum with Plummer softening over all pairs (G = 1).
-void launch_direct64(int64_t n, const StateArrays& st, double eps, double* ax, double* ay, double* az, double* pot,
-                     hipStream_t s);
-// Kinetic energy, potential energy (0.5 sum m phi) and momentum, reduced in
-// a fixed order: out5 (device) = {ekin, epot, px, py, pz}.  part holds
-// 5 * kEnergyBlocks doubles.
-constexpr int kEnergyBlocks = 512;
-void launch_energy(int64_t n, const StateArrays& st, const double* pot, double* part, double* out5, hipStream_t s);
+// Group walk over the group table: accelerations (and potentials if
+// pot != null), optionally followed by the closing half kick.  fp64 selects
+// the FP64 evaluation.
+void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
+                      const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
+                      DevScalars* sc, bool kick, bool count, bool fp64, hipStream_t s);
+// FP64 direct sum with Plummer softening over all pairs of each system (G = 1).
+void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,
+                     double* az, double* pot, hipStream_t s);
+// Kinetic energy, potential energy (0.5 sum m phi) and momentum of each
+// system, reduced in a fixed order over the bounding-box blocks:
+// out (device, 5 per system) = {ekin, epot, px, py, pz}; part holds 5 doubles
+// per block.
+void launch_energy(const StateArrays& st, const double* pot, const BoxBlock* blocks, int nblocks,
+                   const int32_t* block_off, int nsys, double* part, double* out, hipStream_t s);
 
 }  // namespace nbx

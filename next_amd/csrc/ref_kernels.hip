@@ -60,46 +60,52 @@ __global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int wha
     if (t < kMaxDepthSlots) sc->level_count[t] = 0;
     if (t == 0) { sc->collisions = 0; sc->coincident = 0; sc->max_leaf_depth = 0; }
   }
-  if ((what & 2) && t == 0) sc->v2max = 0ull;
   if ((what & 4) && t == 0) sc->err = 0;
   if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; }
 }
 
-__global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
+__global__ __launch_bounds__(kBlock) void k_fill_v2max(SysArrays sy) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < sy.nsys) sy.v2max[s] = 0ull;
+}
 
-__global__ __launch_bounds__(kBlock) void k_bbox_partial(int64_t n, const double* __restrict__ x,
-                                                         const double* __restrict__ y,
-                                                         const double* __restrict__ z, double* part) {
+__global__ __launch_bounds__(kBlock) void k_set_dt(SysArrays sy, double dt) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < sy.nsys) sy.dt[s] = dt;
+}
+
+// Min/max of the bodies [begin, end) of one system (one block).
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(StateArrays st, const BoxBlock* __restrict__ blocks,
+                                                         double* part) {
+  const BoxBlock b = blocks[blockIdx.x];
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const double p[3] = {x[i], y[i], z[i]};
+  for (int64_t i = b.begin + threadIdx.x; i < b.end; i += kBlock) {
+    const double p[3] = {st.x[i], st.y[i], st.z[i]};
     for (int k = 0; k < 3; ++k) { lo_upd(lo[k], p[k]); hi_upd(hi[k], p[k]); }
   }
   block_bbox_write(lo, hi, part);
 }
 
-// Root centre and half-width from the partials (step.h:272-279).
-__global__ __launch_bounds__(kBlock) void k_bbox_final(const double* __restrict__ part, int nparts, DevScalars* sc) {
+// Root centre and half-width of each system from its blocks' partials
+// (step.h:272-279); min and max are exact, so the order does not matter.
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const double* __restrict__ part,
+                                                       const int32_t* __restrict__ block_off, SysArrays sy) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= sy.nsys) return;
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
-  for (int b = threadIdx.x; b < nparts; b += kBlock)
+  for (int b = block_off[s]; b < block_off[s + 1]; ++b)
     for (int k = 0; k < 3; ++k) { lo_upd(lo[k], part[6 * b + k]); hi_upd(hi[k], part[6 * b + 3 + k]); }
-  __shared__ double fin[6];
-  __syncthreads();
-  // reuse the block reducer into a one-entry partial
-  block_bbox_write(lo, hi, fin - 6 * (int64_t)blockIdx.x);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double cx = (fin[0] + fin[3]) * 0.5;
-    const double cy = (fin[1] + fin[4]) * 0.5;
-    const double cz = (fin[2] + fin[5]) * 0.5;
-    double s = fin[3] - fin[0];
-    const double ey = fin[4] - fin[1], ez = fin[5] - fin[2];
-    if (s < ey) s = ey;
-    if (s < ez) s = ez;
-    s = s * 0.5;
-    if (s <= 0.0) s = 1.0;
-    sc->root[0] = cx; sc->root[1] = cy; sc->root[2] = cz; sc->root[3] = s;
-  }
+  const double cx = (lo[0] + hi[0]) * 0.5;
+  const double cy = (lo[1] + hi[1]) * 0.5;
+  const double cz = (lo[2] + hi[2]) * 0.5;
+  double e = hi[0] - lo[0];
+  const double ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  if (e < ey) e = ey;
+  if (e < ez) e = ez;
+  double h = e * 0.5;
+  if (h <= 0.0) h = 1.0;
+  double* r = sy.root + 4 * s;
+  r[0] = cx; r[1] = cy; r[2] = cz; r[3] = h;
 }
 
 // Digits of one body along the node-centre recurrence (octree.h:49-64),
@@ -121,36 +127,44 @@ __device__ __forceinline__ void key_digits(double px, double py, double pz, cons
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_keys(int64_t n, StateArrays st, const DevScalars* __restrict__ sc,
+__device__ __forceinline__ void sys_root(const SysArrays& sy, int32_t s, double root[4]) {
+  const double* r = sy.root + 4 * s;
+  root[0] = r[0]; root[1] = r[1]; root[2] = r[2]; root[3] = r[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_keys(int64_t n, StateArrays st, SysArrays sy,
                                                  uint64_t* __restrict__ key, uint32_t* __restrict__ idx,
                                                  DevScalars* scw) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const double px = st.x[i], py = st.y[i], pz = st.z[i];
   if (!(__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(pz))) atomicOr(&scw->err, 1);
-  double root[4] = {sc->root[0], sc->root[1], sc->root[2], sc->root[3]};
+  double root[4];
+  sys_root(sy, st.sys[i], root);
   uint64_t hi, lo;
   key_digits(px, py, pz, root, 0, kKeyDigits, hi, lo);
   key[i] = hi;
   idx[i] = (uint32_t)i;
 }
 
-__global__ __launch_bounds__(kBlock) void k_keys2(int64_t n, StateArrays st, const DevScalars* __restrict__ sc,
+__global__ __launch_bounds__(kBlock) void k_keys2(int64_t n, StateArrays st, SysArrays sy,
                                                   uint64_t* __restrict__ key2) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double root[4] = {sc->root[0], sc->root[1], sc->root[2], sc->root[3]};
+  double root[4];
+  sys_root(sy, st.sys[i], root);
   uint64_t hi, lo;
   key_digits(st.x[i], st.y[i], st.z[i], root, kKeyDigits, kMaxLevels, hi, lo);
   key2[i] = lo;
 }
 
-__global__ __launch_bounds__(kBlock) void k_keys_export(int64_t n, StateArrays st, const DevScalars* __restrict__ sc,
+__global__ __launch_bounds__(kBlock) void k_keys_export(int64_t n, StateArrays st, SysArrays sy,
                                                         uint64_t* __restrict__ hi_out,
                                                         uint64_t* __restrict__ lo_out) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  double root[4] = {sc->root[0], sc->root[1], sc->root[2], sc->root[3]};
+  double root[4];
+  sys_root(sy, st.sys[i], root);
   uint64_t hi, lo;
   key_digits(st.x[i], st.y[i], st.z[i], root, 0, kMaxLevels, hi, lo);
   const int32_t o = st.id[i];
@@ -176,7 +190,32 @@ __global__ __launch_bounds__(kBlock) void k_gather_state(int64_t n, const uint32
   const uint32_t p = perm[i];
   b.x[i] = a.x[p]; b.y[i] = a.y[p]; b.z[i] = a.z[p];
   b.vx[i] = a.vx[p]; b.vy[i] = a.vy[p]; b.vz[i] = a.vz[p];
-  b.m[i] = a.m[p]; b.type[i] = a.type[p]; b.id[i] = a.id[p];
+  b.m[i] = a.m[p]; b.type[i] = a.type[p]; b.id[i] = a.id[p]; b.sys[i] = a.sys[p];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_sys(int64_t n, const uint32_t* __restrict__ perm,
+                                                       const int32_t* __restrict__ sys, uint32_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = (uint32_t)sys[perm[i]];
+}
+
+// Map for removing the bodies [cut, cut + len) of one system: slot i of the
+// shortened arrays takes old slot perm[i].
+__global__ __launch_bounds__(kBlock) void k_remove_map(int64_t n_new, int64_t cut, int64_t len,
+                                                       uint32_t* __restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n_new) perm[i] = (uint32_t)(i < cut ? i : i + len);
+}
+
+__global__ __launch_bounds__(kBlock) void k_relabel(int64_t n, int32_t* __restrict__ sys, int32_t removed) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && sys[i] > removed) sys[i] -= 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_f64(int64_t n, const uint32_t* __restrict__ perm,
+                                                       const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
 }
 
 // Number of leading octree digits two keys share (0..21 per word).
@@ -185,9 +224,12 @@ __device__ __forceinline__ int lcp_word(uint64_t a, uint64_t b) {
   return d ? (__clzll((long long)d) - 1) / 3 : kKeyDigits;
 }
 
-__device__ __forceinline__ int lcp_pair(const uint64_t* key, const uint64_t* key2, int64_t i, int& equal) {
-  int l = lcp_word(key[i], key[i + 1]);
+// Bodies of different systems share nothing: -1 separates their trees.
+__device__ __forceinline__ int lcp_pair(const uint64_t* key, const uint64_t* key2, const int32_t* sys, int64_t i,
+                                        int& equal) {
   equal = 0;
+  if (sys[i] != sys[i + 1]) return -1;
+  int l = lcp_word(key[i], key[i + 1]);
   if (l == kKeyDigits) {
     if (key2) {
       l += lcp_word(key2[i], key2[i + 1]);
@@ -205,7 +247,8 @@ __device__ __forceinline__ int lcp_pair(const uint64_t* key, const uint64_t* key
 // writes one row of partial counts (internal nodes per depth, deepest leaf,
 // equal-key pairs) for k_lcp_reduce: plain stores, no global atomics.
 __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t* __restrict__ key,
-                                                      const uint64_t* __restrict__ key2, int32_t* __restrict__ lcp,
+                                                      const uint64_t* __restrict__ key2,
+                                                      const int32_t* __restrict__ sys, int32_t* __restrict__ lcp,
                                                       uint32_t* __restrict__ cnt_s, uint32_t* __restrict__ cnt_e,
                                                       uint32_t* __restrict__ part) {
   __shared__ uint32_t hist[kPartW];
@@ -214,8 +257,8 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t*
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) {
     int eq_i = 0, eq_p = 0;
-    const int li = (i + 1 < n) ? lcp_pair(key, key2, i, eq_i) : -1;
-    const int lp = (i > 0) ? lcp_pair(key, key2, i - 1, eq_p) : -1;
+    const int li = (i + 1 < n) ? lcp_pair(key, key2, sys, i, eq_i) : -1;
+    const int lp = (i > 0) ? lcp_pair(key, key2, sys, i - 1, eq_p) : -1;
     lcp[i] = li;
     cnt_s[i] = 1u + (uint32_t)(li > lp ? li - lp : 0);
     cnt_e[i] = (uint32_t)(lp > li ? lp - li : 0);
@@ -254,10 +297,21 @@ __global__ __launch_bounds__(kBlock) void k_lcp_reduce(const uint32_t* __restric
   }
 }
 
+// Preorder index of each system's root (its first body starts at depth 0)
+// and of the node after its tree.
+__global__ __launch_bounds__(kBlock) void k_sys_roots(SysArrays sy, const uint32_t* __restrict__ pre_off,
+                                                      const DevScalars* __restrict__ sc) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= sy.nsys) return;
+  sy.sroot[s] = (int32_t)pre_off[sy.soff[s]];
+  sy.send[s] = s + 1 < sy.nsys ? (int32_t)pre_off[sy.soff[s + 1]] : sc->total_nodes;
+}
+
 __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __restrict__ lcp,
                                                  const uint32_t* __restrict__ pre_off,
                                                  const uint32_t* __restrict__ end_off, StateArrays st,
-                                                 const DevScalars* __restrict__ sc, RefNode* __restrict__ nodes,
+                                                 SysArrays sy, const DevScalars* __restrict__ sc,
+                                                 RefNode* __restrict__ nodes,
                                                  uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
                                                  uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -265,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
   const int li = lcp[i];
   const int lp = i > 0 ? lcp[i - 1] : -1;
   const uint32_t pre = pre_off[i];
-  const double s0 = sc->root[3];
+  const double s0 = sy.root[4 * st.sys[i] + 3];
   const int ns = li > lp ? li - lp : 0;
   const uint32_t sbase = pre - (uint32_t)i;
   for (int t = 0; t < ns; ++t) {
@@ -399,11 +453,17 @@ __device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes,
 // form are below 1e-15), and by the exact expression inside the band, so the
 // decision is NEXT's bit for bit and opened nodes cost no sqrt or divide.
 template <bool KICK, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const RefNode* __restrict__ nodes,
-                                                 StateArrays st, double theta, double* __restrict__ ax,
-                                                 double* __restrict__ ay, double* __restrict__ az, DevScalars* sc) {
-  const int64_t i = xcd_block(gridDim.x) * kBlock + threadIdx.x;
-  const bool valid = i < n;
+__global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ groups, int64_t ngroups,
+                                                 const RefNode* __restrict__ nodes, StateArrays st, SysArrays sy,
+                                                 double theta, double* __restrict__ ax, double* __restrict__ ay,
+                                                 double* __restrict__ az, DevScalars* sc) {
+  // four waves per block, one group per wave
+  const int64_t gi = xcd_block(gridDim.x) * (kBlock / kWave) + threadIdx.x / kWave;
+  if (gi >= ngroups) return;                               // whole wave past the end
+  const Group g = groups[gi];
+  const int64_t i = g.first + lane_id();
+  const bool valid = lane_id() < g.count;
+  const int32_t total = sy.send[g.sys];                    // this system's tree: [sroot, send)
   double px = 0, py = 0, pz = 0, pm = 1;
   int pt = 0;
   if (valid) { px = st.x[i]; py = st.y[i]; pz = st.z[i]; pm = st.m[i]; pt = st.type[i]; }
@@ -411,9 +471,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
   const double inv_theta = 1.0 / theta;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   uint32_t npp = 0, npc = 0, nop = 0, nit = 0;
-  int32_t next = valid ? 0 : INT_MAX;
-  int32_t cur = __ballot(valid) ? 0 : total;
-  NodeHead h = load_head(nodes, 0);
+  const int32_t root = sy.sroot[g.sys];
+  int32_t next = valid ? root : INT_MAX;
+  int32_t cur = root;
+  NodeHead h = load_head(nodes, root);
   while (cur < total) {
     // successors, loaded while this node is processed
     const int32_t sk = h.skip > cur ? h.skip : cur + 1;   // skip > cur by construction
@@ -481,11 +542,12 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
     if (__ballot(open)) { cur = cur + 1; h = h1; }
     else { cur = sk; h = h2; }
   }
+  (void)h;
   double v2 = 0.0;
   if (valid) {
     ax[i] = a0; ay[i] = a1; az[i] = a2;
     if (KICK) {                                             // step.h:350-352
-      const double half = sc->dt * 0.5;
+      const double half = sy.dt[g.sys] * 0.5;
       const double vx = st.vx[i] + a0 * half;
       const double vy = st.vy[i] + a1 * half;
       const double vz = st.vz[i] + a2 * half;
@@ -500,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
       const double o = __shfl_xor(v2, off, kWave);
       if (o > v2) v2 = o;
     }
-    if (lane_id() == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
+    if (lane_id() == 0 && v2 > 0.0) atomicMax(&sy.v2max[g.sys], (unsigned long long)__double_as_longlong(v2));
   }
   if (COUNT) {
     const uint32_t sp = wave_sum_u32(npp), sc2 = wave_sum_u32(npc), so = wave_sum_u32(nop);
@@ -513,46 +575,53 @@ __global__ __launch_bounds__(kBlock) void k_walk(int64_t n, int32_t total, const
   }
 }
 
-// Opening half kick and drift (step.h:294-325), with the partial bounding
-// box of the drifted positions for the next tree build.
+// Opening half kick and drift (step.h:294-325), each system with its dt.
 __global__ __launch_bounds__(kBlock) void k_kick_drift(int64_t n, StateArrays st, const double* __restrict__ ax,
                                                        const double* __restrict__ ay, const double* __restrict__ az,
-                                                       const DevScalars* __restrict__ sc, double* part) {
-  const double dt = sc->dt;
+                                                       SysArrays sy) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double dt = sy.dt[st.sys[i]];
   const double half = dt * 0.5;
-  double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const double vx = st.vx[i] + ax[i] * half;
-    const double vy = st.vy[i] + ay[i] * half;
-    const double vz = st.vz[i] + az[i] * half;
-    const double x = st.x[i] + vx * dt;
-    const double y = st.y[i] + vy * dt;
-    const double z = st.z[i] + vz * dt;
-    st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
-    st.x[i] = x; st.y[i] = y; st.z[i] = z;
-    lo_upd(lo[0], x); lo_upd(lo[1], y); lo_upd(lo[2], z);
-    hi_upd(hi[0], x); hi_upd(hi[1], y); hi_upd(hi[2], z);
-  }
-  block_bbox_write(lo, hi, part);
+  const double vx = st.vx[i] + ax[i] * half;
+  const double vy = st.vy[i] + ay[i] * half;
+  const double vz = st.vz[i] + az[i] * half;
+  st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
+  st.x[i] = st.x[i] + vx * dt;
+  st.y[i] = st.y[i] + vy * dt;
+  st.z[i] = st.z[i] + vz * dt;
 }
 
-__global__ __launch_bounds__(kBlock) void k_maxv2(int64_t n, StateArrays st, DevScalars* sc) {
+// Max |v|^2 per system (adaptive.h:31-37).  A wave whose bodies belong to one
+// system makes one atomic; a wave spanning systems, one per lane.
+__global__ __launch_bounds__(kBlock) void k_maxv2(int64_t n, StateArrays st, SysArrays sy) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double v2 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const double s = st.vx[i] * st.vx[i] + st.vy[i] * st.vy[i] + st.vz[i] * st.vz[i];
-    if (s > v2) v2 = s;                                     // adaptive.h:34-37
+  int32_t s = -1;
+  if (i < n) {
+    v2 = st.vx[i] * st.vx[i] + st.vy[i] * st.vy[i] + st.vz[i] * st.vz[i];
+    if (!(v2 > 0.0)) v2 = 0.0;
+    s = st.sys[i];
+  }
+  const int32_t s0 = __shfl(s, 0, kWave);
+  if (__ballot(s != s0 && s >= 0)) {
+    if (s >= 0 && v2 > 0.0) atomicMax(&sy.v2max[s], (unsigned long long)__double_as_longlong(v2));
+    return;
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const double o = __shfl_xor(v2, off, kWave);
     if (o > v2) v2 = o;
   }
-  if (lane_id() == 0 && v2 > 0.0) atomicMax(&sc->v2max, (unsigned long long)__double_as_longlong(v2));
+  if (lane_id() == 0 && s0 >= 0 && v2 > 0.0) atomicMax(&sy.v2max[s0], (unsigned long long)__double_as_longlong(v2));
 }
 
-// computeAdaptiveDt (adaptive.h:38-51) from the maximum |v|^2.
-__global__ void k_dt(DevScalars* sc, double base, double* dts, int64_t step) {
-  const double v2 = __longlong_as_double((long long)sc->v2max);
+// computeAdaptiveDt (adaptive.h:38-51) for each system from its max |v|^2;
+// dts (may be null) receives system 0's value for `step`.
+__global__ __launch_bounds__(kBlock) void k_dt(SysArrays sy, double base, double* dts, int64_t step) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= sy.nsys) return;
+  const double v2 = __longlong_as_double((long long)sy.v2max[s]);
   double v = sqrt(v2);
   if (1e4 < v) v = 1e4;
   double dt;
@@ -563,9 +632,8 @@ __global__ void k_dt(DevScalars* sc, double base, double* dts, int64_t step) {
     if (dt < base * 0.01) dt = base * 0.01;
     if (base * 1.0 < dt) dt = base * 1.0;
   }
-  sc->dt = dt;
-  sc->base_dt = base;
-  if (dts) dts[step] = dt;
+  sy.dt[s] = dt;
+  if (dts && s == 0) dts[step] = dt;
 }
 
 __device__ __forceinline__ uint32_t digit_at(const uint64_t* key, const uint64_t* key2, int32_t f, int l) {
@@ -575,7 +643,7 @@ __device__ __forceinline__ uint32_t digit_at(const uint64_t* key, const uint64_t
 
 __global__ __launch_bounds__(kBlock) void k_export(int64_t total, const RefNode* __restrict__ nodes,
                                                    const uint64_t* __restrict__ key, const uint64_t* __restrict__ key2,
-                                                   const int32_t* __restrict__ id, const DevScalars* __restrict__ sc,
+                                                   const int32_t* __restrict__ id, SysArrays sy,
                                                    int32_t* depth, int32_t* leaf, int32_t* body, int32_t* cmask,
                                                    double* geo4, double* mass, double* com3, double* q6) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -584,7 +652,7 @@ __global__ __launch_bounds__(kBlock) void k_export(int64_t total, const RefNode*
   depth[k] = nd.depth;
   leaf[k] = nd.body >= 0 ? 1 : 0;
   body[k] = nd.body >= 0 ? id[nd.body] : -1;
-  double cx = sc->root[0], cy = sc->root[1], cz = sc->root[2], s = sc->root[3];
+  double cx = sy.root[0], cy = sy.root[1], cz = sy.root[2], s = sy.root[3];
   for (int l = 0; l < nd.depth; ++l) {
     const uint32_t o = digit_at(key, key2, nd.first, l);
     const double hs = s * 0.5;
@@ -610,30 +678,32 @@ inline unsigned grid_of(int64_t n) { return (unsigned)div_up(n, kBlock); }
 
 }  // namespace
 
+inline unsigned sys_grid(const SysArrays& sy) { return (unsigned)div_up(std::max(sy.nsys, 1), kBlock); }
+
 void launch_fill_scalars(DevScalars* sc, int what, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_scalars, dim3(1), dim3(kBlock), 0, s, sc, what);
 }
-void launch_set_dt(DevScalars* sc, double dt, hipStream_t s) {
-  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, s, sc, dt);
+void launch_fill_v2max(const SysArrays& sy, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_v2max, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy);
 }
-void launch_bbox(int64_t n, const StateArrays& st, double* part, DevScalars* sc, hipStream_t s) {
-  const int nb = (int)std::min<int64_t>(std::max<int64_t>(div_up(n, kBlock), 1), kBboxBlocks);
-  hipLaunchKernelGGL(k_bbox_partial, dim3(nb), dim3(kBlock), 0, s, n, st.x, st.y, st.z, part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, part, nb, sc);
+void launch_set_dt(const SysArrays& sy, double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_dt, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, dt);
 }
-void launch_bbox_final(const double* part, int nparts, DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, s, part, nparts, sc);
+void launch_bbox(const StateArrays& st, const BoxBlock* blocks, int nblocks, const int32_t* block_off, double* part,
+                 const SysArrays& sy, hipStream_t s) {
+  if (nblocks > 0) hipLaunchKernelGGL(k_bbox_partial, dim3(nblocks), dim3(kBlock), 0, s, st, blocks, part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(sys_grid(sy)), dim3(kBlock), 0, s, part, block_off, sy);
 }
-void launch_keys(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* key, uint32_t* idx,
+void launch_keys(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint32_t* idx,
                  DevScalars* scw, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sc, key, idx, scw);
+  if (n > 0) hipLaunchKernelGGL(k_keys, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, key, idx, scw);
 }
-void launch_keys2(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* key2, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_keys2, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sc, key2);
+void launch_keys2(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key2, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_keys2, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, key2);
 }
-void launch_keys_export(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* hi, uint64_t* lo,
+void launch_keys_export(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* hi, uint64_t* lo,
                         hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_keys_export, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sc, hi, lo);
+  if (n > 0) hipLaunchKernelGGL(k_keys_export, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, hi, lo);
 }
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(kBlock), 0, s, n, idx);
@@ -641,22 +711,38 @@ void launch_iota(int64_t n, uint32_t* idx, hipStream_t s) {
 void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uint64_t* dst, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_gather_u64, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
 }
+void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint32_t* dst, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_sys, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, sys, dst);
+}
 void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
                          hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_gather_state, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
 }
-void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, int32_t* lcp, uint32_t* cnt_s,
-                      uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s) {
+void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, const int32_t* sys, int32_t* lcp,
+                      uint32_t* cnt_s, uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s) {
   if (n <= 0) return;
   const int64_t rows = grid_of(n);
-  hipLaunchKernelGGL(k_lcp_count, dim3((unsigned)rows), dim3(kBlock), 0, s, n, key, key2, lcp, cnt_s, cnt_e, part);
+  hipLaunchKernelGGL(k_lcp_count, dim3((unsigned)rows), dim3(kBlock), 0, s, n, key, key2, sys, lcp, cnt_s, cnt_e,
+                     part);
   hipLaunchKernelGGL(k_lcp_reduce, dim3(kPartW), dim3(kBlock), 0, s, part, rows, key2 ? 1 : 0, sc);
 }
+void launch_remove_map(int64_t n_new, int64_t cut, int64_t len, uint32_t* perm, hipStream_t s) {
+  if (n_new > 0) hipLaunchKernelGGL(k_remove_map, dim3(grid_of(n_new)), dim3(kBlock), 0, s, n_new, cut, len, perm);
+}
+void launch_relabel(int64_t n, int32_t* sys, int32_t removed, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_relabel, dim3(grid_of(n)), dim3(kBlock), 0, s, n, sys, removed);
+}
+void launch_gather_f64(int64_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_f64, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
+}
+void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_sys_roots, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, pre_off, sc);
+}
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
-                 const StateArrays& st, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
+                 const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
                  uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sc, nodes,
+    hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sy, sc, nodes,
                        st_key, st_val, en_key, en_val);
 }
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
@@ -666,37 +752,35 @@ void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefN
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, hipStream_t s) {
   if (cnt > 0) hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes);
 }
-void launch_walk(int64_t n, int64_t total, const RefNode* nodes, const StateArrays& st, double theta, double* ax,
-                 double* ay, double* az, DevScalars* sc, bool kick, bool count, hipStream_t s) {
-  if (n <= 0) return;
-  const unsigned nb = (unsigned)(div_up(grid_of(n), 8) * 8);
-  const int32_t t = (int32_t)total;
-  if (kick && count)
-    hipLaunchKernelGGL((k_walk<true, true>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
-  else if (kick)
-    hipLaunchKernelGGL((k_walk<true, false>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
-  else if (count)
-    hipLaunchKernelGGL((k_walk<false, true>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
-  else
-    hipLaunchKernelGGL((k_walk<false, false>), dim3(nb), dim3(kBlock), 0, s, n, t, nodes, st, theta, ax, ay, az, sc);
+void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
+                 const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
+                 bool count, hipStream_t s) {
+  if (ngroups <= 0) return;
+  const int wpb = kBlock / kWave;
+  const unsigned nb = (unsigned)(div_up(div_up(ngroups, wpb), 8) * 8);
+#define NBX_RW(K, C) \
+  hipLaunchKernelGGL((k_walk<K, C>), dim3(nb), dim3(kBlock), 0, s, groups, ngroups, nodes, st, sy, theta, ax, ay, az, sc)
+  if (kick && count) NBX_RW(true, true);
+  else if (kick) NBX_RW(true, false);
+  else if (count) NBX_RW(false, true);
+  else NBX_RW(false, false);
+#undef NBX_RW
 }
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
-                       const DevScalars* sc, double* bbox_part, hipStream_t s) {
-  const int nb = (int)std::min<int64_t>(std::max<int64_t>(div_up(n, kBlock), 1), kBboxBlocks);
-  hipLaunchKernelGGL(k_kick_drift, dim3(nb), dim3(kBlock), 0, s, n, st, ax, ay, az, sc, bbox_part);
+                       const SysArrays& sy, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_kick_drift, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, ax, ay, az, sy);
 }
-void launch_maxv2(int64_t n, const StateArrays& st, DevScalars* sc, hipStream_t s) {
-  const int nb = (int)std::min<int64_t>(std::max<int64_t>(div_up(n, kBlock), 1), kBboxBlocks);
-  hipLaunchKernelGGL(k_maxv2, dim3(nb), dim3(kBlock), 0, s, n, st, sc);
+void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_maxv2, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy);
 }
-void launch_dt(DevScalars* sc, double base, double* dts, int64_t step, hipStream_t s) {
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, s, sc, base, dts, step);
+void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, base, dts, step);
 }
 void launch_export(int64_t total, const RefNode* nodes, const uint64_t* key, const uint64_t* key2,
-                   const int32_t* id, const DevScalars* sc, int32_t* depth, int32_t* leaf, int32_t* body,
+                   const int32_t* id, const SysArrays& sy, int32_t* depth, int32_t* leaf, int32_t* body,
                    int32_t* cmask, double* geo4, double* mass, double* com3, double* q6, hipStream_t s) {
   if (total > 0)
-    hipLaunchKernelGGL(k_export, dim3(grid_of(total)), dim3(kBlock), 0, s, total, nodes, key, key2, id, sc, depth,
+    hipLaunchKernelGGL(k_export, dim3(grid_of(total)), dim3(kBlock), 0, s, total, nodes, key, key2, id, sy, depth,
                        leaf, body, cmask, geo4, mass, com3, q6);
 }
 

@@ -10,49 +10,56 @@
 namespace nbx {
 
 // Particle state, structure of arrays (particle.h:22-52) plus `id`, the
-// upload-order index of the body held in each slot.
+// upload-order index of the body held in each slot, and `sys`, its system.
 struct StateArrays {
   double *x, *y, *z, *vx, *vy, *vz, *m;
-  int32_t *type, *id;
+  int32_t *type, *id, *sys;
 };
 
 constexpr int kBlock = 256;
-constexpr int kBboxBlocks = 1024;   // partial min/max blocks (grid-stride)
+constexpr int kBoxSpan = 8192;      // bodies per block of the bounding-box reduction
 // Row of per-block partial counts written by the lcp kernel.
 constexpr int kPartMaxDepth = kMaxDepthSlots;      // column: deepest leaf
 constexpr int kPartEqual = kMaxDepthSlots + 1;     // column: equal-key pairs
 constexpr int kPartW = kMaxDepthSlots + 2;
 inline int64_t lcp_part_words(int64_t n) { return ((n + kBlock - 1) / kBlock) * kPartW; }
 
-void launch_fill_scalars(DevScalars* sc, int what, hipStream_t s);  // what: 1 build counters, 2 v2max, 4 err
-void launch_set_dt(DevScalars* sc, double dt, hipStream_t s);
-void launch_bbox(int64_t n, const StateArrays& st, double* part, DevScalars* sc, hipStream_t s);
-void launch_keys(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* key, uint32_t* idx,
+void launch_fill_scalars(DevScalars* sc, int what, hipStream_t s);  // what: 1 build counters, 4 err, 8 walk counters
+void launch_fill_v2max(const SysArrays& sy, hipStream_t s);
+void launch_set_dt(const SysArrays& sy, double dt, hipStream_t s);
+void launch_bbox(const StateArrays& st, const BoxBlock* blocks, int nblocks, const int32_t* block_off, double* part,
+                 const SysArrays& sy, hipStream_t s);
+void launch_keys(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint32_t* idx,
                  DevScalars* scw, hipStream_t s);
-void launch_keys2(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* key2, hipStream_t s);
+void launch_keys2(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key2, hipStream_t s);
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s);
 void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uint64_t* dst, hipStream_t s);
+void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint32_t* dst, hipStream_t s);
 void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
                          hipStream_t s);
-void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, int32_t* lcp, uint32_t* cnt_s,
-                      uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s);
+void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, const int32_t* sys, int32_t* lcp,
+                      uint32_t* cnt_s, uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s);
+void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevScalars* sc, hipStream_t s);
+void launch_remove_map(int64_t n_new, int64_t cut, int64_t len, uint32_t* perm, hipStream_t s);
+void launch_relabel(int64_t n, int32_t* sys, int32_t removed, hipStream_t s);
+void launch_gather_f64(int64_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s);
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
-                 const StateArrays& st, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
+                 const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
                  uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, hipStream_t s);
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
                  hipStream_t s);
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, hipStream_t s);
-void launch_walk(int64_t n, int64_t total, const RefNode* nodes, const StateArrays& st, double theta, double* ax,
-                 double* ay, double* az, DevScalars* sc, bool kick, bool count, hipStream_t s);
+void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
+                 const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
+                 bool count, hipStream_t s);
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
-                       const DevScalars* sc, double* bbox_part, hipStream_t s);
-void launch_bbox_final(const double* part, int nparts, DevScalars* sc, hipStream_t s);
-void launch_maxv2(int64_t n, const StateArrays& st, DevScalars* sc, hipStream_t s);
-void launch_dt(DevScalars* sc, double base, double* dts, int64_t step, hipStream_t s);
+                       const SysArrays& sy, hipStream_t s);
+void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStream_t s);
+void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, hipStream_t s);
 void launch_export(int64_t total, const RefNode* nodes, const uint64_t* key, const uint64_t* key2,
-                   const int32_t* id, const DevScalars* sc, int32_t* depth, int32_t* leaf, int32_t* body,
+                   const int32_t* id, const SysArrays& sy, int32_t* depth, int32_t* leaf, int32_t* body,
                    int32_t* cmask, double* geo4, double* mass, double* com3, double* q6, hipStream_t s);
-void launch_keys_export(int64_t n, const StateArrays& st, const DevScalars* sc, uint64_t* hi, uint64_t* lo,
+void launch_keys_export(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* hi, uint64_t* lo,
                         hipStream_t s);
 
 }  // namespace nbx

@@ -1,0 +1,307 @@
+// Continuous-batching request loop over the engine (nbx_server_* in nbx.h).
+//
+// A request is one independent simulation: its bodies, a step count and its
+// settings.  Requests are admitted first come, first served while their
+// estimated device memory fits the budget.  Admitted requests with the same
+// settings share one batch context (one HIP stream, one host thread): a
+// request joins its batch between two steps, every step advances all of the
+// batch's systems with one set of kernel launches, and a request leaves its
+// batch as soon as it has made its steps.  Batches with different settings
+// run on their own streams, so their work overlaps on the device.  A
+// system's trajectory does not depend on the other systems of its batch, so
+// a request's result is the same alone and in any mix.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "nbx.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0, Clock::time_point t1) {
+  return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+// Estimated device bytes per body (state, keys, tree nodes of both modes).
+constexpr size_t kBytesPerBody = 700;
+
+struct Request {
+  int64_t id = 0;
+  nbx_request spec{};
+  std::vector<double> x, y, z, vx, vy, vz, m;
+  std::vector<int32_t> type;
+  size_t bytes = 0;
+  int32_t steps_done = 0;
+  int state = 0;            // 0 queued, 1 running, 2 done, 3 failed
+  nbx_status status = NBX_OK;
+  std::string error;
+  Clock::time_point t_submit, t_start, t_done;
+};
+
+struct BatchKey {
+  int mode, precision, leaf, adaptive;
+  double theta, eps, dt;
+  bool operator<(const BatchKey& o) const {
+    if (mode != o.mode) return mode < o.mode;
+    if (precision != o.precision) return precision < o.precision;
+    if (leaf != o.leaf) return leaf < o.leaf;
+    if (adaptive != o.adaptive) return adaptive < o.adaptive;
+    if (theta != o.theta) return theta < o.theta;
+    if (eps != o.eps) return eps < o.eps;
+    return dt < o.dt;
+  }
+};
+
+struct Batch {
+  BatchKey key{};
+  nbx_ctx* ctx = nullptr;
+  std::vector<Request*> members;    // index = system index in ctx
+  std::deque<Request*> incoming;
+  std::thread thread;
+};
+
+}  // namespace
+
+struct nbx_server {
+  int device = 0;
+  size_t budget = 0;
+  size_t in_use = 0;
+  std::mutex mu;
+  std::condition_variable cv;       // state changes: submissions, completions, stop
+  std::deque<Request*> pending;
+  std::map<int64_t, std::unique_ptr<Request>> requests;
+  std::map<BatchKey, std::unique_ptr<Batch>> batches;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread scheduler;
+  std::atomic<int64_t> body_steps{0};
+  std::atomic<int64_t> batch_steps{0};
+};
+
+namespace {
+
+void finish(nbx_server* s, Request* r, nbx_status st, const std::string& err) {
+  r->status = st;
+  r->error = err;
+  r->state = st == NBX_OK ? 2 : 3;
+  r->t_done = Clock::now();
+  s->in_use -= r->bytes;
+}
+
+// One batch: admit its newcomers, step everyone once, retire the finished.
+void batch_loop(nbx_server* s, Batch* b) {
+  for (;;) {
+    std::vector<Request*> joining;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stop || !b->incoming.empty() || !b->members.empty(); });
+      if (s->stop) return;
+      joining.assign(b->incoming.begin(), b->incoming.end());
+      b->incoming.clear();
+    }
+    for (Request* r : joining) {
+      int32_t sys = -1;
+      const nbx_status st = nbx_add_system(b->ctx, r->spec.n, r->x.data(), r->y.data(), r->z.data(), r->vx.data(),
+                                           r->vy.data(), r->vz.data(), r->m.data(), r->type.data(), &sys);
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (st != NBX_OK) {
+        finish(s, r, st, nbx_last_error(b->ctx));
+        s->cv.notify_all();
+        continue;
+      }
+      r->state = 1;
+      r->t_start = Clock::now();
+      b->members.push_back(r);
+    }
+    if (b->members.empty()) continue;
+    int64_t bodies = 0;
+    for (Request* r : b->members) bodies += r->spec.n;
+    const nbx_status st = nbx_run(b->ctx, b->key.dt, 1, nullptr);
+    if (st != NBX_OK) {
+      const std::string err = nbx_last_error(b->ctx);
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (Request* r : b->members) finish(s, r, st, err);
+      b->members.clear();
+      (void)nbx_upload_batch(b->ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr);
+      s->cv.notify_all();
+      continue;
+    }
+    s->body_steps += bodies;
+    s->batch_steps += 1;
+    for (Request* r : b->members) r->steps_done += 1;
+    // retire finished systems, highest index first so the others keep theirs
+    for (int32_t k = (int32_t)b->members.size() - 1; k >= 0; --k) {
+      Request* r = b->members[(size_t)k];
+      if (r->steps_done < r->spec.nsteps) continue;
+      const nbx_status rs = nbx_remove_system(b->ctx, k, r->x.data(), r->y.data(), r->z.data(), r->vx.data(),
+                                              r->vy.data(), r->vz.data());
+      std::lock_guard<std::mutex> lk(s->mu);
+      finish(s, r, rs, rs == NBX_OK ? std::string() : std::string(nbx_last_error(b->ctx)));
+      b->members.erase(b->members.begin() + k);
+      s->cv.notify_all();
+    }
+  }
+}
+
+// Admission: first come, first served, while the estimated memory fits.
+void scheduler_loop(nbx_server* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv.wait(lk, [&] {
+      return s->stop || (!s->pending.empty() && s->in_use + s->pending.front()->bytes <= s->budget);
+    });
+    if (s->stop) return;
+    while (!s->pending.empty() && s->in_use + s->pending.front()->bytes <= s->budget) {
+      Request* r = s->pending.front();
+      s->pending.pop_front();
+      const BatchKey key{r->spec.mode, r->spec.precision, r->spec.leaf_size, r->spec.adaptive_dt, r->spec.theta,
+                         r->spec.eps, r->spec.dt};
+      auto it = s->batches.find(key);
+      if (it == s->batches.end()) {
+        auto b = std::make_unique<Batch>();
+        b->key = key;
+        nbx_config cfg;
+        nbx_config_default(&cfg);
+        cfg.device = s->device;
+        cfg.mode = key.mode;
+        cfg.precision = key.precision;
+        cfg.theta = key.theta;
+        cfg.eps = key.eps;
+        cfg.leaf_size = key.leaf;
+        cfg.adaptive_dt = key.adaptive;
+        const nbx_status st = nbx_create(&b->ctx, &cfg);
+        if (st != NBX_OK) {
+          s->in_use += r->bytes;
+          finish(s, r, st, "could not create a context for this request's settings");
+          s->cv.notify_all();
+          continue;
+        }
+        Batch* bp = b.get();
+        it = s->batches.emplace(key, std::move(b)).first;
+        bp->thread = std::thread(batch_loop, s, bp);
+      }
+      s->in_use += r->bytes;
+      it->second->incoming.push_back(r);
+      s->cv.notify_all();
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+nbx_status nbx_server_create(nbx_server** out, int device, uint64_t budget_bytes) {
+  if (!out || budget_bytes == 0) return NBX_ERR_ARG;
+  nbx_server* s = new nbx_server;
+  s->device = device;
+  s->budget = (size_t)budget_bytes;
+  s->scheduler = std::thread(scheduler_loop, s);
+  *out = s;
+  return NBX_OK;
+}
+
+void nbx_server_destroy(nbx_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->scheduler.joinable()) s->scheduler.join();
+  for (auto& kv : s->batches) {
+    if (kv.second->thread.joinable()) kv.second->thread.join();
+    nbx_destroy(kv.second->ctx);
+  }
+  delete s;
+}
+
+nbx_status nbx_server_submit(nbx_server* s, const nbx_request* rq, int64_t* id) {
+  if (!s || !rq || !id) return NBX_ERR_ARG;
+  const nbx_request& q = *rq;
+  if (q.n <= 0 || q.nsteps < 0 || !(q.dt > 0.0)) return NBX_ERR_ARG;
+  if (!(q.x && q.y && q.z && q.vx && q.vy && q.vz && q.m && q.type)) return NBX_ERR_ARG;
+  if (q.mode != NBX_MODE_REFERENCE && q.mode != NBX_MODE_PHYSICS) return NBX_ERR_ARG;
+  auto r = std::make_unique<Request>();
+  r->spec = q;
+  const size_t n = (size_t)q.n;
+  r->x.assign(q.x, q.x + n); r->y.assign(q.y, q.y + n); r->z.assign(q.z, q.z + n);
+  r->vx.assign(q.vx, q.vx + n); r->vy.assign(q.vy, q.vy + n); r->vz.assign(q.vz, q.vz + n);
+  r->m.assign(q.m, q.m + n);
+  r->type.assign(q.type, q.type + n);
+  r->spec.x = r->spec.y = r->spec.z = r->spec.vx = r->spec.vy = r->spec.vz = r->spec.m = nullptr;
+  r->spec.type = nullptr;
+  r->bytes = n * kBytesPerBody;
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (r->bytes > s->budget) return NBX_ERR_NOMEM;   // could never be admitted
+  r->id = s->next_id++;
+  *id = r->id;
+  Request* rp = r.get();
+  s->requests.emplace(r->id, std::move(r));
+  if (q.nsteps == 0) {
+    s->in_use += rp->bytes;
+    finish(s, rp, NBX_OK, "");
+  } else {
+    s->pending.push_back(rp);
+  }
+  s->cv.notify_all();
+  return NBX_OK;
+}
+
+int nbx_server_poll(nbx_server* s, int64_t id) {
+  if (!s) return NBX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->requests.find(id);
+  if (it == s->requests.end()) return NBX_ERR_ARG;
+  return it->second->state;
+}
+
+nbx_status nbx_server_wait(nbx_server* s, int64_t id, double* x, double* y, double* z, double* vx, double* vy,
+                           double* vz, nbx_request_timing* timing) {
+  if (!s) return NBX_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->requests.find(id);
+  if (it == s->requests.end()) return NBX_ERR_ARG;
+  Request* r = it->second.get();
+  s->cv.wait(lk, [&] { return r->state >= 2; });
+  if (timing) {
+    timing->queue_ms = r->state == 2 && r->spec.nsteps > 0 ? ms_since(r->t_submit, r->t_start) : 0.0;
+    timing->latency_ms = ms_since(r->t_submit, r->t_done);
+  }
+  if (r->state == 3) return r->status;
+  const size_t n = (size_t)r->spec.n;
+  double* outs[6] = {x, y, z, vx, vy, vz};
+  const std::vector<double>* src[6] = {&r->x, &r->y, &r->z, &r->vx, &r->vy, &r->vz};
+  for (int k = 0; k < 6; ++k)
+    if (outs[k]) std::memcpy(outs[k], src[k]->data(), n * sizeof(double));
+  s->requests.erase(it);   // the result has been handed over
+  return NBX_OK;
+}
+
+nbx_status nbx_server_stats(nbx_server* s, nbx_server_info* out) {
+  if (!s || !out) return NBX_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  out->queued = (int64_t)s->pending.size();
+  int64_t running = 0;
+  for (auto& kv : s->batches) running += (int64_t)kv.second->members.size() + (int64_t)kv.second->incoming.size();
+  out->running = running;
+  out->batches = (int64_t)s->batches.size();
+  out->bytes_in_use = (uint64_t)s->in_use;
+  out->body_steps = s->body_steps.load();
+  out->batch_steps = s->batch_steps.load();
+  return NBX_OK;
+}
+
+}  // extern "C"

@@ -25,7 +25,9 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
            "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
-           "nbx_potential", "nbx_direct", "nbx_energy"]
+           "nbx_potential", "nbx_direct", "nbx_energy", "nbx_energy_systems", "nbx_upload_batch", "nbx_add_system",
+           "nbx_download_system", "nbx_remove_system", "nbx_num_systems", "nbx_system_size", "nbx_server_create",
+           "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats"]
 
 
 class NbxError(RuntimeError):
@@ -44,6 +46,21 @@ class Timing(C.Structure):
                 ("upward_ms", C.c_double), ("walk_ms", C.c_double), ("kick_ms", C.c_double),
                 ("total_ms", C.c_double), ("nodes", C.c_int64), ("depth", C.c_int32), ("n_pp", C.c_int64),
                 ("n_pc", C.c_int64), ("n_open", C.c_int64), ("n_iter", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in ("x", "y", "z", "vx", "vy", "vz", "m", "type")] + [
+        ("nsteps", C.c_int32), ("dt", C.c_double), ("mode", C.c_int), ("precision", C.c_int), ("theta", C.c_double),
+        ("eps", C.c_double), ("leaf_size", C.c_int), ("adaptive_dt", C.c_int)]
+
+
+class RequestTiming(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_int64), ("running", C.c_int64), ("batches", C.c_int64), ("bytes_in_use", C.c_uint64),
+                ("body_steps", C.c_int64), ("batch_steps", C.c_int64)]
 
 
 _lib = None
@@ -86,6 +103,23 @@ def lib():
     L.nbx_potential.argtypes = [VP, f64p]
     L.nbx_direct.argtypes = [VP, VP, VP, VP, VP]
     L.nbx_energy.argtypes = [VP, C.c_int, f64p]
+    L.nbx_energy_systems.argtypes = [VP, C.c_int, f64p]
+    i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    L.nbx_upload_batch.argtypes = [VP, C.c_int32, i64p] + [f64p] * 7 + [i32p]
+    L.nbx_add_system.argtypes = [VP, C.c_int64] + [f64p] * 7 + [i32p, C.POINTER(C.c_int32)]
+    L.nbx_download_system.argtypes = [VP, C.c_int32] + [f64p] * 6
+    L.nbx_remove_system.argtypes = [VP, C.c_int32] + [f64p] * 6
+    L.nbx_num_systems.argtypes = [VP]
+    L.nbx_num_systems.restype = C.c_int32
+    L.nbx_system_size.argtypes = [VP, C.c_int32]
+    L.nbx_system_size.restype = C.c_int64
+    L.nbx_server_create.argtypes = [C.POINTER(VP), C.c_int, C.c_uint64]
+    L.nbx_server_destroy.argtypes = [VP]
+    L.nbx_server_destroy.restype = None
+    L.nbx_server_submit.argtypes = [VP, C.POINTER(Request), C.POINTER(C.c_int64)]
+    L.nbx_server_poll.argtypes = [VP, C.c_int64]
+    L.nbx_server_wait.argtypes = [VP, C.c_int64] + [f64p] * 6 + [C.POINTER(RequestTiming)]
+    L.nbx_server_stats.argtypes = [VP, C.POINTER(ServerStats)]
     _lib = L
     return L
 
@@ -143,6 +177,47 @@ class Engine:
             t = np.zeros(1, np.int32)
         self._check(lib().nbx_upload(self._h, n, *arrs, t))
         self.n = n
+
+    def upload_batch(self, systems):
+        """Replace the contents with several independent systems."""
+        counts = np.array([len(p) for p in systems], dtype=np.int64)
+        allp = Particles.concat(systems)
+        arrs = [_f64(a) for a in (allp.x, allp.y, allp.z, allp.vx, allp.vy, allp.vz, allp.m)]
+        self._check(lib().nbx_upload_batch(self._h, len(systems), counts, *arrs,
+                                           np.ascontiguousarray(allp.type, dtype=np.int32)))
+        self.n = int(counts.sum())
+
+    def add_system(self, p: Particles) -> int:
+        s = C.c_int32()
+        arrs = [_f64(a) for a in (p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m)]
+        self._check(lib().nbx_add_system(self._h, len(p), *arrs, np.ascontiguousarray(p.type, dtype=np.int32),
+                                         C.byref(s)))
+        self.n += len(p)
+        return s.value
+
+    def num_systems(self) -> int:
+        return lib().nbx_num_systems(self._h)
+
+    def system_size(self, s: int) -> int:
+        return lib().nbx_system_size(self._h, s)
+
+    def download_system(self, s: int):
+        k = self.system_size(s)
+        out = [np.zeros(max(k, 1)) for _ in range(6)]
+        self._check(lib().nbx_download_system(self._h, s, *out))
+        return np.stack([o[:k] for o in out])
+
+    def remove_system(self, s: int):
+        k = self.system_size(s)
+        out = [np.zeros(max(k, 1)) for _ in range(6)]
+        self._check(lib().nbx_remove_system(self._h, s, *out))
+        self.n -= k
+        return np.stack([o[:k] for o in out])
+
+    def energy_systems(self, direct: bool = False):
+        out = np.zeros(5 * max(self.num_systems(), 1))
+        self._check(lib().nbx_energy_systems(self._h, 1 if direct else 0, out))
+        return out[: 5 * self.num_systems()].reshape(-1, 5)
 
     def download(self):
         out = [np.zeros(max(self.n, 1)) for _ in range(6)]
@@ -217,3 +292,60 @@ class Engine:
         out = {key: v[:k] for key, v in t.items() if key != "perm"}
         out["perm"] = t["perm"][: self.n]
         return out
+
+
+class Server:
+    """Continuous-batching request loop (nbx_server_* in include/nbx.h)."""
+
+    def __init__(self, budget_bytes: int, device: int = 0):
+        h = VP()
+        st = lib().nbx_server_create(C.byref(h), device, int(budget_bytes))
+        if st != 0:
+            raise NbxError(st, "nbx_server_create failed")
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nbx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, p: Particles, nsteps: int, dt: float, mode: int = MODE_PHYSICS, precision: int = PREC_MIXED,
+               theta: float = 0.5, eps: float = 0.01, leaf_size: int = 16, adaptive_dt: bool = False) -> int:
+        arrs = [_f64(a) for a in (p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m)]
+        t = np.ascontiguousarray(p.type, dtype=np.int32)
+        r = Request(len(p), *[a.ctypes.data_as(C.c_void_p) for a in arrs], t.ctypes.data_as(C.c_void_p), nsteps, dt,
+                    mode, precision, theta, eps, leaf_size, 1 if adaptive_dt else 0)
+        rid = C.c_int64()
+        st = lib().nbx_server_submit(self._h, C.byref(r), C.byref(rid))
+        if st != 0:
+            raise NbxError(st, "submit refused")
+        self._n[rid.value] = len(p)
+        return rid.value
+
+    def poll(self, rid: int) -> int:
+        return lib().nbx_server_poll(self._h, rid)
+
+    def wait(self, rid: int):
+        """(state 6 x n, queue_ms, latency_ms)."""
+        n = self._n.pop(rid)
+        out = [np.zeros(max(n, 1)) for _ in range(6)]
+        tm = RequestTiming()
+        st = lib().nbx_server_wait(self._h, rid, *out, C.byref(tm))
+        if st != 0:
+            raise NbxError(st, f"request {rid} failed")
+        return np.stack([o[:n] for o in out]), tm.queue_ms, tm.latency_ms
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        lib().nbx_server_stats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}

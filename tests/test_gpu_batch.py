@@ -1,0 +1,97 @@
+"""Several independent systems in one context: every system's trajectory is
+bit-identical to the same system run in a context of its own, in reference
+mode and in physics mode, including systems added and removed mid-run."""
+import numpy as np
+import pytest
+
+from next_amd import ic
+from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def systems():
+    a = ic.plummer(3000, seed=1, type_fraction_dm=0.2)
+    b = ic.uniform_sphere(5000, seed=2)
+    b.vx += 0.05
+    c = ic.plummer(700, seed=3)
+    c.x += 3.0
+    return [a, b, c]
+
+
+def make(mode, adaptive):
+    if mode == "reference":
+        return Engine(theta=0.5, adaptive_dt=adaptive)
+    return Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=adaptive)
+
+
+@pytest.mark.parametrize("mode", ["reference", "physics"])
+@pytest.mark.parametrize("adaptive", [True, False])
+def test_batch_equals_alone(mode, adaptive):
+    sy = systems()
+    alone = []
+    for p in sy:
+        with make(mode, adaptive) as e:
+            e.upload(p)
+            e.run(1e-3, 5)
+            alone.append(e.download())
+    with make(mode, adaptive) as e:
+        e.upload_batch(sy)
+        assert e.num_systems() == 3
+        e.run(1e-3, 5)
+        for s, ref in enumerate(alone):
+            assert np.array_equal(e.download_system(s), ref), s
+        allst = e.download()
+    assert np.array_equal(allst, np.concatenate(alone, axis=1))
+
+
+@pytest.mark.parametrize("mode", ["reference", "physics"])
+def test_add_and_remove_mid_run(mode):
+    a, b, c = systems()
+
+    def alone(p, nsteps):
+        with make(mode, True) as e:
+            e.upload(p)
+            e.run(1e-3, nsteps)
+            return e.download()
+
+    with make(mode, True) as e:
+        e.upload_batch([a, b])
+        e.run(1e-3, 3)
+        sc = e.add_system(c)
+        assert sc == 2
+        e.run(1e-3, 2)
+        out_a = e.remove_system(0)           # a: 5 steps; b moves to index 0, c to 1
+        assert e.num_systems() == 2
+        e.run(1e-3, 1)
+        out_b = e.download_system(0)         # b: 6 steps
+        out_c = e.download_system(1)         # c: 3 steps
+    assert np.array_equal(out_a, alone(a, 5))
+    assert np.array_equal(out_b, alone(b, 6))
+    assert np.array_equal(out_c, alone(c, 3))
+
+
+def test_energy_per_system():
+    sy = systems()
+    with make("physics", False) as e:
+        e.upload_batch(sy)
+        per = e.energy_systems(direct=True)
+    for s, p in enumerate(sy):
+        with make("physics", False) as e:
+            e.upload(p)
+            one = e.energy(direct=True)
+        assert np.allclose(per[s], one, rtol=1e-12, atol=1e-15)
+
+
+def test_many_small_systems():
+    sizes = ic.ensemble_sizes(64, lo=64, hi=2048)
+    sy = [ic.plummer(n, seed=100 + i) for i, n in enumerate(sizes)]
+    with make("physics", False) as e:
+        e.upload_batch(sy)
+        e.run(1e-3, 3)
+        got = [e.download_system(s) for s in (0, 17, 63)]
+    for s, g in zip((0, 17, 63), got):
+        with make("physics", False) as e:
+            e.upload(sy[s])
+            e.run(1e-3, 3)
+            assert np.array_equal(g, e.download())
