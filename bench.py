@@ -37,6 +37,10 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_PP, FLOPS_PC = 19, 55
 
 
+def log(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_threads() -> int:
     n = os.cpu_count() or 1
     return max(1, min(16, n))
@@ -72,6 +76,63 @@ def cpu_baseline(p: ic.Particles, base_dt: float, theta: float, steps: int) -> d
     return {"value": ms, "unit": "ms per KDK step", "cores": threads, "kind": "port", "sample": sample}
 
 
+def ensemble(nsys: int, steps: int, warmup: int) -> dict:
+    """Batched ensemble: nsys Plummer spheres (a = 1, M = 1, G = 1, eps = 0.01;
+    samples of the distribution function, not rescaled to an exact virial
+    ratio), N_i log-uniform in [1024, 65536] from seed i, theta = 0.5,
+    dt = 1e-3, FP32 walk with FP64 state, all advanced by one set of launches
+    per step."""
+    from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
+
+    sizes = ic.ensemble_sizes(nsys)
+    systems = [ic.plummer(n, seed=i, virial=None) for i, n in enumerate(sizes)]
+    with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False) as e:
+        e.upload_batch(systems)
+        e.run(1e-3, warmup)
+        t0 = time.perf_counter()
+        e.run(1e-3, steps)
+        dt = time.perf_counter() - t0
+    total = int(sum(sizes))
+    return {"systems": nsys, "bodies": total, "steps": steps, "ensemble_steps_per_s": steps / dt,
+            "ms_per_step": dt * 1e3 / steps, "body_steps_per_s": total * steps / dt,
+            "workload": f"{nsys} Plummer spheres (DF samples), N log-uniform [1024, 65536], theta 0.5 (bmax criterion), "
+                        "eps 0.01, dt 1e-3, FP32 walk / FP64 state"}
+
+
+def serving(nreq: int, rate: float, seed: int = 5) -> dict:
+    """Serving sample: Poisson arrivals at `rate` per second of independent
+    Plummer or cold-sphere runs, N log-uniform in [4096, 131072], 20-60 steps,
+    theta from {0.4, 0.5, 0.7}, FP32 or FP64 walk, 200 GB budget."""
+    from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Server
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    reqs = []
+    for k in range(nreq):
+        n = int(round(np.exp(rng.uniform(np.log(4096), np.log(131072)))))
+        p = (ic.plummer(n, seed=1000 + k, virial=None) if rng.uniform() < 0.5
+             else ic.uniform_sphere(n, seed=1000 + k))
+        reqs.append(dict(p=p, nsteps=int(rng.integers(20, 61)), theta=float(rng.choice([0.4, 0.5, 0.7])),
+                         precision=PREC_MIXED if rng.uniform() < 0.5 else PREC_FP64,
+                         gap=float(rng.exponential(1.0 / rate))))
+    lat, ids = [], []
+    with Server(budget_bytes=200 << 30) as srv:
+        t0 = time.perf_counter()
+        for r in reqs:
+            time.sleep(r["gap"])
+            ids.append(srv.submit(r["p"], r["nsteps"], 1e-3, mode=MODE_PHYSICS, precision=r["precision"],
+                                  theta=r["theta"], eps=0.01, leaf_size=16))
+        for i in ids:
+            _, _, l_ms = srv.wait(i)
+            lat.append(l_ms)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    return {"requests": nreq, "arrival_rate_per_s": rate, "p50_latency_ms": float(np.percentile(lat, 50)),
+            "p99_latency_ms": float(np.percentile(lat, 99)), "body_steps_per_s": st["body_steps"] / wall,
+            "batches": st["batches"],
+            "workload": "Poisson arrivals; Plummer (DF sample) or cold sphere, N log-uniform [4096, 131072], 20-60 steps, "
+                        "theta {0.4, 0.5, 0.7}, FP32/FP64 walk, dt 1e-3, eps 0.01, 200 GB budget"}
+
+
 def load_traffic() -> dict | None:
     path = os.path.join(ROOT, "profiles", "walk_traffic.json")
     if not os.path.exists(path):
@@ -90,7 +151,21 @@ def main() -> int:
     ap.add_argument("--base-dt", type=float, default=5e-4)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and serving samples")
+    ap.add_argument("--ensemble-systems", type=int, default=512)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB device buffer 3 times (scale for HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(1 << 26, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}))
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -127,7 +202,7 @@ def main() -> int:
     traffic = load_traffic()
 
     out = {
-        "metric": "ms per KDK step at N=1M, theta=0.5",
+        "metric": "ms per KDK step at N=1M, \u03b8=0.5; ensemble steps/s + p50/p99 latency",
         "value": round(ms_per_step, 4),
         "unit": "ms/step",
         "n_gpus": 1,
@@ -150,13 +225,24 @@ def main() -> int:
                      "walk_ms": round(walk_ms, 4), "flops_per_launch": flops,
                      "n_pp": cnt["n_pp"], "n_pc": cnt["n_pc"]},
         "interactions_per_s": (cnt["n_pp"] + cnt["n_pc"]) / (walk_ms * 1e-3) if walk_ms > 0 else 0.0,
-        "walk_counters": {"n_open": cnt["n_open"], "wave_iterations": cnt["n_iter"],
-                          "lane_efficiency": round((cnt["n_pp"] + cnt["n_pc"] + cnt["n_open"])
-                                                   / max(64 * cnt["n_iter"], 1), 4)},
+        "walk_counters": {"n_open": cnt["n_open"], "traversal_wave_steps": cnt["n_iter"],
+                          "traversal_lane_efficiency": round((cnt["n_pp"] + cnt["n_pc"] + cnt["n_open"])
+                                                             / max(64 * cnt["n_iter"], 1), 4),
+                          "evaluation_wave_steps": cnt["n_eval"],
+                          "evaluation_lane_efficiency": round((cnt["n_pp"] + cnt["n_pc"])
+                                                              / max(64 * cnt["n_eval"], 1), 4)},
         "phases_ms": {key: round(tim[key] / k, 4) for key in
                       ("kick_ms", "keys_ms", "sort_ms", "build_ms", "upward_ms", "walk_ms", "total_ms")},
         "tree": {"nodes": tim["nodes"], "depth": tim["depth"]},
     }
+    log("headline: %.3f ms/step" % ms_per_step)
+    if not args.no_extra:
+        out["ensemble"] = ensemble(args.ensemble_systems, steps=5, warmup=1)
+        log("ensemble: %.1f ms/step" % out["ensemble"]["ms_per_step"])
+        out["serving"] = []
+        for rate in (50.0, 200.0):
+            out["serving"].append(serving(args.serve_requests, rate))
+            log("serving at %g/s: p50 %.1f ms" % (rate, out["serving"][-1]["p50_latency_ms"]))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.base_dt, args.theta, args.cpu_steps)
     print(json.dumps(out))

@@ -90,8 +90,9 @@ typedef struct nbx_timing {
   int32_t depth;       /* deepest leaf of the last build */
   int64_t n_pp;        /* particle-particle interactions (if counting) */
   int64_t n_pc;        /* particle-cell interactions (if counting) */
-  int64_t n_open;      /* opened cells (if counting) */
-  int64_t n_iter;      /* node visits summed over waves (if counting) */
+  int64_t n_open;      /* opened cells, per target (reference) or per group (physics) (if counting) */
+  int64_t n_iter;      /* walk traversal steps summed over waves (if counting) */
+  int64_t n_eval;      /* interaction-evaluation steps summed over waves (if counting) */
 } nbx_timing;
 
 /* Fill *cfg with NEXT's settings: device 0, reference mode, FP64, theta 0.5,

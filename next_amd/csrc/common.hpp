@@ -41,7 +41,8 @@ struct DevScalars {
   int32_t max_leaf_depth;
   int32_t level_count[kMaxDepthSlots];  // internal nodes per depth
   unsigned long long n_pp, n_pc;        // interaction counters
-  unsigned long long n_open, n_iter;    // opened nodes, wave iterations
+  unsigned long long n_open, n_iter;    // opened nodes, wave traversal iterations
+  unsigned long long n_eval;            // wave evaluation steps (deferred interactions)
 };
 
 // Per-system arrays: a context holds one or more independent particle

@@ -403,6 +403,7 @@ nbx_status finish_timing(nbx_ctx* c, bool with_kick) {
     c->tim.n_pc += (int64_t)c->hsc->n_pc;
     c->tim.n_open += (int64_t)c->hsc->n_open;
     c->tim.n_iter += (int64_t)c->hsc->n_iter;
+    c->tim.n_eval += (int64_t)c->hsc->n_eval;
   }
   c->tim.steps += 1;
   c->tim.nodes = c->total_nodes;

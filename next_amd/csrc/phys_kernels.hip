@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   const T e2 = (T)(eps * eps);
 
   double Ax = 0, Ay = 0, Az = 0, P = 0;
-  uint32_t ncell_tot = 0, npart_tot = 0;
+  uint32_t ncell_tot = 0, npart_tot = 0, niter = 0, nopen = 0;
   int nc = 0, np = 0;
 
   auto flush_cells = [&]() {
@@ -283,6 +283,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
     sp -= take;
     const bool busy = cur < end;
     if (!__ballot(busy)) break;
+    if (COUNT) ++niter;
     if (nc > kPhysCells - kWave) flush_cells();
 
     bool acc = false, bucket = false, donate = false;
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
     }
     sp += (int)__popcll(dm) < room ? (int)__popcll(dm) : room;
     if (busy) cur = (acc || bucket) ? nskip : cur + 1;
+    if (COUNT) nopen += (uint32_t)__popcll(__ballot(busy && !acc && !bucket));
 
     // buckets -> body list, all lanes loading bodies in parallel
     const uint32_t cb_n = bucket ? (uint32_t)cnt : 0u;
@@ -378,6 +380,8 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
     const uint32_t nv = (uint32_t)g.count;
     atomicAdd(&sc->n_pc, (unsigned long long)ncell_tot * nv);
     atomicAdd(&sc->n_pp, (unsigned long long)npart_tot * nv);
+    atomicAdd(&sc->n_open, (unsigned long long)nopen);
+    atomicAdd(&sc->n_iter, (unsigned long long)niter);
   }
 }
 

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int wha
     if (t == 0) { sc->collisions = 0; sc->coincident = 0; sc->max_leaf_depth = 0; }
   }
   if ((what & 4) && t == 0) sc->err = 0;
-  if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; }
+  if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; sc->n_eval = 0ull; }
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_v2max(SysArrays sy) {
@@ -425,9 +425,17 @@ __device__ __forceinline__ int64_t xcd_block(int nb) {
   return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// Softening floor of a dark-matter target (octree.h:146-148), out of line:
+// expanded in place, the FP64 pow raises the walk's register count by about
+// half for every target.
+__device__ __noinline__ double dm_softening(double eps, double size, double m, double mi) {
+  const double alt = 2.0 * size / pow(m / mi, 0.333333333);
+  return eps < alt ? alt : eps;
+}
+
 // The per-visit fields of a node, held in scalar registers.
 struct NodeHead {
-  double cx, cy, cz, m, size, esum;
+  double cx, cy, cz, m, size;
   int32_t skip, body;
 };
 
@@ -435,17 +443,27 @@ __device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes,
   const RefNode* p = nodes + k;
   NodeHead h;
   h.cx = p->cx; h.cy = p->cy; h.cz = p->cz; h.m = p->m;
-  h.size = p->size; h.esum = p->esum; h.skip = p->skip; h.body = p->body;
+  h.size = p->size; h.skip = p->skip; h.body = p->body;
   return h;
 }
 
-// Union traversal.  Every lane follows its own depth-first path (open: go to
-// the first child, cur+1; accept or skip: go past the subtree, skip[cur]).
-// The wave visits node `cur` and the lanes whose own next node is `cur`
-// process it.  No lane that is waiting can point inside cur's subtree (its
-// pointer came from a skip of an ancestor or an earlier sibling, both of
-// which end at or after skip[cur]), so the wave's next node is cur+1 when
-// any lane opened cur and skip[cur] otherwise: one ballot, no reduction.
+// Accepted nodes a lane collects before the wave evaluates them.
+constexpr int kWalkList = 24;
+
+// Two phases per wave.  Traversal: every lane follows its own depth-first
+// path (open: go to the first child, cur+1; accept or skip: go past the
+// subtree, skip[cur]).  The wave visits node `cur` and the lanes whose own
+// next node is `cur` decide it.  No lane that is waiting can point inside
+// cur's subtree (its pointer came from a skip of an ancestor or an earlier
+// sibling, both of which end at or after skip[cur]), so the wave's next node
+// is cur+1 when any lane opened cur and skip[cur] otherwise: one ballot, no
+// reduction.  An accepted node is only appended to the lane's list in LDS.
+// Evaluation: when some lane's list is full (and at the end), the wave runs
+// the lists in step, entry k of every lane at once, so the costly
+// interaction (two square roots, two divides, the quadrupole) runs with most
+// lanes busy instead of only the lanes that accepted the node being visited.
+// Each lane still adds its interactions in its own depth-first order, the
+// order of octree.h:158-177, so the sums are NEXT's bit for bit.
 //
 // Opening test of octree.h:153, (size/dist) < theta with
 // dist = sqrt(r2 + 1e-20): decided from r2 + 1e-20 against (size/theta)^2
@@ -457,9 +475,15 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
                                                  const RefNode* __restrict__ nodes, StateArrays st, SysArrays sy,
                                                  double theta, double* __restrict__ ax, double* __restrict__ ay,
                                                  double* __restrict__ az, DevScalars* sc) {
-  // four waves per block, one group per wave
-  const int64_t gi = xcd_block(gridDim.x) * (kBlock / kWave) + threadIdx.x / kWave;
+  __shared__ int32_t lists[kBlock / kWave][kWalkList][kWave];
+  // Four waves per block, one group per wave.  The wave index goes through
+  // readfirstlane so the compiler sees it as wave-uniform: the group, its
+  // system's tree bounds and every node visited are then uniform too, and
+  // the node records come in through scalar loads into scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int64_t gi = xcd_block(gridDim.x) * (kBlock / kWave) + wave;
   if (gi >= ngroups) return;                               // whole wave past the end
+  int32_t* const mine = &lists[wave][0][lane_id()];      // entry k at mine[k * kWave]
   const Group g = groups[gi];
   const int64_t i = g.first + lane_id();
   const bool valid = lane_id() < g.count;
@@ -470,7 +494,51 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   const int32_t me = valid ? (int32_t)i : -2;
   const double inv_theta = 1.0 / theta;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  uint32_t npp = 0, npc = 0, nop = 0, nit = 0;
+  uint32_t npp = 0, npc = 0, nop = 0, nit = 0, nev = 0;
+  int len = 0;
+
+  // Evaluate the collected entries (octree.h:139-176 for an accepted node);
+  // an entry is the node index, with bit 31 set for a leaf (Q = 0).
+  auto evaluate = [&]() {
+    for (int k = 0; __ballot(k < len); ++k) {
+      if (COUNT) ++nev;
+      if (k < len) {
+        const int32_t e = mine[k * kWave];
+        const bool leaf = e < 0;
+        const RefNode* nd = nodes + (e & 0x7fffffff);
+        const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
+        const double m = nd->m;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double dist = sqrt(r2 + 1e-20);
+        double eps = nd->esum * (1.0 / (1.0 + dist * 10.0));  // softening.h:26-36
+        if (eps < 1e-4) eps = 1e-4;
+        if (pt == 1) eps = dm_softening(eps, nd->size, m, pm);   // octree.h:146-148
+        const double r2s = r2 + eps * eps;
+        const double dinv = 1.0 / sqrt(r2s);
+        const double inv3 = dinv * dinv * dinv;
+        const double fac = m * inv3;                        // G = 1: (1.0 * m) * inv3
+        a0 = a0 + dx * fac;
+        a1 = a1 + dy * fac;
+        a2 = a2 + dz * fac;
+        if (!leaf) {                                        // leaves carry Q = 0
+          const double qxx = nd->qxx, qyy = nd->qyy, qzz = nd->qzz;
+          const double qxy = nd->qxy, qxz = nd->qxz, qyz = nd->qyz;
+          const double inv5 = inv3 * (dinv * dinv);
+          const double inv7 = inv5 * (dinv * dinv);
+          const double q = qxx * dx * dx + qyy * dy * dy + qzz * dz * dz +
+                           2.0 * (qxy * dx * dy + qxz * dx * dz + qyz * dy * dz);
+          const double qrx = 2.0 * (qxx * dx + qxy * dy + qxz * dz);
+          const double qry = 2.0 * (qxy * dx + qyy * dy + qyz * dz);
+          const double qrz = 2.0 * (qxz * dx + qyz * dy + qzz * dz);
+          a0 = a0 + 0.5 * (qrx * inv5 - 5.0 * q * inv7 * dx);
+          a1 = a1 + 0.5 * (qry * inv5 - 5.0 * q * inv7 * dy);
+          a2 = a2 + 0.5 * (qrz * inv5 - 5.0 * q * inv7 * dz);
+        }
+      }
+    }
+    len = 0;
+  };
+
   const int32_t root = sy.sroot[g.sys];
   int32_t next = valid ? root : INT_MAX;
   int32_t cur = root;
@@ -485,12 +553,11 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
       if (h.m == 0.0 || h.body == me) {                    // octree.h:134-135
         next = sk;
       } else {
-        const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double r2e = r2 + 1e-20;
         const bool leaf = h.body >= 0;
         bool accept = leaf;
         if (!leaf) {
+          const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
+          const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
           const double q = h.size * inv_theta;
           const double q2 = q * q;
           if (r2e > q2 * (1.0 + 1e-12)) accept = true;
@@ -498,39 +565,10 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
           else accept = (h.size / sqrt(r2e)) < theta;
         }
         if (accept) {
-          const double dist = sqrt(r2e);
-          double eps = h.esum * (1.0 / (1.0 + dist * 10.0));  // softening.h:26-36
-          if (eps < 1e-4) eps = 1e-4;
-          if (pt == 1) {                                    // octree.h:146-148
-            const double alt = 2.0 * h.size / pow(h.m / pm, 0.333333333);
-            if (eps < alt) eps = alt;
-          }
-          const double r2s = r2 + eps * eps;
-          const double dinv = 1.0 / sqrt(r2s);
-          const double inv3 = dinv * dinv * dinv;
-          const double fac = h.m * inv3;                    // G = 1: (1.0 * m) * inv3
-          a0 = a0 + dx * fac;
-          a1 = a1 + dy * fac;
-          a2 = a2 + dz * fac;
-          if (!leaf) {                                      // leaves carry Q = 0
-            const RefNode* nd = nodes + cur;
-            const double qxx = nd->qxx, qyy = nd->qyy, qzz = nd->qzz;
-            const double qxy = nd->qxy, qxz = nd->qxz, qyz = nd->qyz;
-            const double inv5 = inv3 * (dinv * dinv);
-            const double inv7 = inv5 * (dinv * dinv);
-            const double q = qxx * dx * dx + qyy * dy * dy + qzz * dz * dz +
-                             2.0 * (qxy * dx * dy + qxz * dx * dz + qyz * dy * dz);
-            const double qrx = 2.0 * (qxx * dx + qxy * dy + qxz * dz);
-            const double qry = 2.0 * (qxy * dx + qyy * dy + qyz * dz);
-            const double qrz = 2.0 * (qxz * dx + qyz * dy + qzz * dz);
-            a0 = a0 + 0.5 * (qrx * inv5 - 5.0 * q * inv7 * dx);
-            a1 = a1 + 0.5 * (qry * inv5 - 5.0 * q * inv7 * dy);
-            a2 = a2 + 0.5 * (qrz * inv5 - 5.0 * q * inv7 * dz);
-            if (COUNT) ++npc;
-          } else {
-            if (COUNT) ++npp;
-          }
+          mine[len * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
+          ++len;
           next = sk;
+          if (COUNT) { if (leaf) ++npp; else ++npc; }
         } else {
           open = true;                                      // first child
           next = cur + 1;
@@ -541,7 +579,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
     if (COUNT) ++nit;
     if (__ballot(open)) { cur = cur + 1; h = h1; }
     else { cur = sk; h = h2; }
+    if (__ballot(len == kWalkList)) evaluate();
   }
+  evaluate();
   (void)h;
   double v2 = 0.0;
   if (valid) {
@@ -571,6 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
       atomicAdd(&sc->n_pc, (unsigned long long)sc2);
       atomicAdd(&sc->n_open, (unsigned long long)so);
       atomicAdd(&sc->n_iter, (unsigned long long)nit);
+      atomicAdd(&sc->n_eval, (unsigned long long)nev);
     }
   }
 }

@@ -45,7 +45,8 @@ class Timing(C.Structure):
     _fields_ = [("steps", C.c_int64), ("keys_ms", C.c_double), ("sort_ms", C.c_double), ("build_ms", C.c_double),
                 ("upward_ms", C.c_double), ("walk_ms", C.c_double), ("kick_ms", C.c_double),
                 ("total_ms", C.c_double), ("nodes", C.c_int64), ("depth", C.c_int32), ("n_pp", C.c_int64),
-                ("n_pc", C.c_int64), ("n_open", C.c_int64), ("n_iter", C.c_int64)]
+                ("n_pc", C.c_int64), ("n_open", C.c_int64), ("n_iter", C.c_int64),
+                ("n_eval", C.c_int64)]
 
 
 class Request(C.Structure):
