@@ -195,6 +195,14 @@ nbx_status nbx_debug_tree(nbx_ctx* ctx, int32_t* depth, int32_t* leaf, int32_t* 
                           int32_t* cmask, double* geo4, double* mass, double* com3, double* q6,
                           int32_t* perm);
 
+/* The walk groups of the current positions, at most cap of them: first
+ * body (an index into the leaf order of nbx_debug_tree's perm), body count
+ * and system.  *ngroups receives the full count.  Reference mode: runs of
+ * 64 consecutive bodies of a system; physics mode: groups packed from the
+ * tree (at most group_size bodies inside one node). */
+nbx_status nbx_debug_groups(nbx_ctx* ctx, int64_t cap, int32_t* first, int32_t* count, int32_t* sys,
+                            int64_t* ngroups);
+
 /* ---- serving: a continuous-batching request loop -----------------------
  * No counterpart in NEXT.  Each request is an independent simulation with
  * its own settings.  Requests are admitted first come, first served while

@@ -43,6 +43,8 @@ struct DevScalars {
   unsigned long long n_pp, n_pc;        // interaction counters
   unsigned long long n_open, n_iter;    // opened nodes, wave traversal iterations
   unsigned long long n_eval;            // wave evaluation steps (deferred interactions)
+  uint32_t tree_groups;                 // walk groups built from the tree (physics mode)
+  uint32_t pad_;
 };
 
 // Per-system arrays: a context holds one or more independent particle

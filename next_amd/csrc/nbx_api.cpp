@@ -65,6 +65,8 @@ struct nbx_ctx {
   DBuf s_root, s_dt, s_v2, s_soff, s_sroot, s_send;
   DBuf groups, boxes, box_off, box_part;
   int64_t ngroups = 0;
+  DBuf tgroups, tgcnt;        // physics mode: walk groups from the tree, per-node counts
+  int64_t ntgroups = 0;
   int nboxes = 0;
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
@@ -328,7 +330,8 @@ nbx_status build_tree(nbx_ctx* c) {
   for (DBuf* b : {&c->st_key, &c->st_val, &c->en_key, &c->en_val, &c->st_key2, &c->st_val2, &c->en_key2,
                   &c->en_val2, &c->level_list})
     NBXCHK(ensure(c, *b, mm));
-  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(std::max(c->cap, m)), scan_scratch_words(c->cap)) * 4));
+  NBXCHK(ensure(c, c->scratch,
+                std::max(sort_scratch_words(std::max(c->cap, m)), scan_scratch_words(std::max(c->cap, total))) * 4));
   scr = P<uint32_t>(c->scratch);
   RefNode* nodes = P<RefNode>(c->nodes);
   launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, sy, c->sc, nodes,
@@ -351,6 +354,12 @@ nbx_status build_tree(nbx_ctx* c) {
       if (c->hsc->level_count[d] > 0)
         launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<PhysNode>(c->pnodes),
                            c->paux.p, n, total, c->cfg.theta, s);
+    NBXCHK(ensure(c, c->tgroups, (size_t)n * sizeof(Group)));
+    NBXCHK(ensure(c, c->tgcnt, (size_t)total * 4));
+    launch_tree_groups(total, n, nodes, St.sys, c->cfg.group_size, P<uint32_t>(c->tgcnt), P<Group>(c->tgroups),
+                       &c->sc->tree_groups, scr, s);
+    NBXCHK(sync_scalars(c));
+    c->ntgroups = c->hsc->tree_groups;
   } else {
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
@@ -367,7 +376,7 @@ nbx_status walk(nbx_ctx* c, bool kick) {
   if (kick) launch_fill_v2max(sysa(c), c->stream);
   record(c, EV_WALK0);
   if (c->physics)
-    launch_phys_walk(P<Group>(c->groups), c->ngroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
+    launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                      c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr,
                      c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64, c->stream);
   else
@@ -518,7 +527,8 @@ nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
     // The group test needs theta < 1 (a cell holding a target is then never
     // accepted for that target's group) and a positive softening length.
     if (!(c0.theta > 0.0 && c0.theta < 1.0) || !(c0.eps > 0.0)) return NBX_ERR_ARG;
-    if (c0.leaf_size < 1 || c0.leaf_size > kPhysMaxLeaf || c0.group_size != kPhysBlock) return NBX_ERR_ARG;
+    if (c0.leaf_size < 1 || c0.leaf_size > kPhysMaxLeaf || c0.group_size < 1 || c0.group_size > kPhysBlock)
+      return NBX_ERR_ARG;
     if (c0.precision != NBX_PREC_FP64 && c0.precision != NBX_PREC_MIXED) return NBX_ERR_ARG;
   }
   int ndev = 0;
@@ -561,7 +571,8 @@ void nbx_destroy(nbx_ctx* c) {
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
                   &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout, &c->s_root, &c->s_dt, &c->s_v2,
-                  &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part})
+                  &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part, &c->tgroups,
+                  &c->tgcnt})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->sc) (void)hipFree(c->sc);
@@ -768,7 +779,7 @@ static nbx_status tree_potential(nbx_ctx* c) {
     NBXCHK(build_tree(c));
     c->acc_valid = false;
   }
-  launch_phys_walk(P<Group>(c->groups), c->ngroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
+  launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                    c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]),
                    P<double>(c->pot), c->sc, false, false, c->cfg.precision == NBX_PREC_FP64, c->stream);
   HIPCHK(c, hipGetLastError());
@@ -887,6 +898,32 @@ nbx_status nbx_debug_tree_size(nbx_ctx* c, int64_t* nodes) {
     c->acc_valid = false;
   }
   *nodes = c->total_nodes;
+  return NBX_OK;
+}
+
+nbx_status nbx_debug_groups(nbx_ctx* c, int64_t cap, int32_t* first, int32_t* count, int32_t* sys,
+                            int64_t* ngroups) {
+  NBXCHK(check_ctx(c));
+  if (!ngroups || cap < 0 || (cap > 0 && !(first && count && sys))) return fail(c, NBX_ERR_ARG, "bad output");
+  *ngroups = 0;
+  if (c->n == 0) return NBX_OK;
+  if (!c->tree_valid) {
+    NBXCHK(build_tree(c));
+    c->acc_valid = false;
+  }
+  const int64_t ng = c->physics ? c->ntgroups : c->ngroups;
+  const Group* dg = P<Group>(c->physics ? c->tgroups : c->groups);
+  *ngroups = ng;
+  const int64_t k = std::min(ng, cap);
+  if (k == 0) return NBX_OK;
+  std::vector<Group> h((size_t)k);
+  HIPCHK(c, hipMemcpyAsync(h.data(), dg, (size_t)k * sizeof(Group), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int64_t j = 0; j < k; ++j) {
+    first[j] = h[(size_t)j].first;
+    count[j] = h[(size_t)j].count;
+    sys[j] = h[(size_t)j].sys;
+  }
   return NBX_OK;
 }
 

@@ -447,8 +447,13 @@ __device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes,
   return h;
 }
 
-// Accepted nodes a lane collects before the wave evaluates them.
-constexpr int kWalkList = 24;
+// Accepted nodes a lane holds (a ring in LDS, a power of two), and how many
+// of each lane's oldest entries one evaluation round takes.  Evaluating only
+// the oldest entries when some lane's ring is full keeps more lanes busy per
+// evaluation step than draining every ring: lanes fill at different rates.
+constexpr int kWalkList = 32;
+constexpr int kWalkFlush = 8;
+static_assert((kWalkList & (kWalkList - 1)) == 0 && kWalkFlush <= kWalkList, "walk ring");
 
 // Two phases per wave.  Traversal: every lane follows its own depth-first
 // path (open: go to the first child, cur+1; accept or skip: go past the
@@ -457,13 +462,14 @@ constexpr int kWalkList = 24;
 // cur's subtree (its pointer came from a skip of an ancestor or an earlier
 // sibling, both of which end at or after skip[cur]), so the wave's next node
 // is cur+1 when any lane opened cur and skip[cur] otherwise: one ballot, no
-// reduction.  An accepted node is only appended to the lane's list in LDS.
-// Evaluation: when some lane's list is full (and at the end), the wave runs
-// the lists in step, entry k of every lane at once, so the costly
-// interaction (two square roots, two divides, the quadrupole) runs with most
-// lanes busy instead of only the lanes that accepted the node being visited.
-// Each lane still adds its interactions in its own depth-first order, the
-// order of octree.h:158-177, so the sums are NEXT's bit for bit.
+// reduction.  An accepted node is only appended to the lane's ring in LDS.
+// Evaluation: when some lane's ring is full, the wave evaluates up to
+// kWalkFlush of every lane's oldest entries in step, entry k of every lane
+// at once (and everything at the end), so the costly interaction (two square
+// roots, two divides, the quadrupole) runs with most lanes busy instead of
+// only the lanes that accepted the node being visited.  Each lane still adds
+// its interactions in its own depth-first order, the order of
+// octree.h:158-177, so the sums are NEXT's bit for bit.
 //
 // Opening test of octree.h:153, (size/dist) < theta with
 // dist = sqrt(r2 + 1e-20): decided from r2 + 1e-20 against (size/theta)^2
@@ -483,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t gi = xcd_block(gridDim.x) * (kBlock / kWave) + wave;
   if (gi >= ngroups) return;                               // whole wave past the end
-  int32_t* const mine = &lists[wave][0][lane_id()];      // entry k at mine[k * kWave]
+  int32_t* const mine = &lists[wave][0][lane_id()];      // ring slot k at mine[k * kWave]
   const Group g = groups[gi];
   const int64_t i = g.first + lane_id();
   const bool valid = lane_id() < g.count;
@@ -495,15 +501,17 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   const double inv_theta = 1.0 / theta;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   uint32_t npp = 0, npc = 0, nop = 0, nit = 0, nev = 0;
-  int len = 0;
+  int head = 0, len = 0;                                   // this lane's ring: [head, head + len)
 
-  // Evaluate the collected entries (octree.h:139-176 for an accepted node);
-  // an entry is the node index, with bit 31 set for a leaf (Q = 0).
-  auto evaluate = [&]() {
-    for (int k = 0; __ballot(k < len); ++k) {
+  // Evaluate up to `limit` of the oldest entries (octree.h:139-176 for an
+  // accepted node); an entry is the node index, with bit 31 set for a leaf
+  // (Q = 0).
+  auto evaluate = [&](int limit) {
+    const int take = len < limit ? len : limit;
+    for (int k = 0; __ballot(k < take); ++k) {
       if (COUNT) ++nev;
-      if (k < len) {
-        const int32_t e = mine[k * kWave];
+      if (k < take) {
+        const int32_t e = mine[((head + k) & (kWalkList - 1)) * kWave];
         const bool leaf = e < 0;
         const RefNode* nd = nodes + (e & 0x7fffffff);
         const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
@@ -536,7 +544,8 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
         }
       }
     }
-    len = 0;
+    head = (head + take) & (kWalkList - 1);
+    len -= take;
   };
 
   const int32_t root = sy.sroot[g.sys];
@@ -565,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
           else accept = (h.size / sqrt(r2e)) < theta;
         }
         if (accept) {
-          mine[len * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
+          mine[((head + len) & (kWalkList - 1)) * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
           ++len;
           next = sk;
           if (COUNT) { if (leaf) ++npp; else ++npc; }
@@ -579,9 +588,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
     if (COUNT) ++nit;
     if (__ballot(open)) { cur = cur + 1; h = h1; }
     else { cur = sk; h = h2; }
-    if (__ballot(len == kWalkList)) evaluate();
+    if (__ballot(len == kWalkList)) evaluate(kWalkFlush);
   }
-  evaluate();
+  evaluate(kWalkList);
   (void)h;
   double v2 = 0.0;
   if (valid) {
@@ -614,6 +623,58 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
       atomicAdd(&sc->n_eval, (unsigned long long)nev);
     }
   }
+}
+
+// Walk groups from the tree.  A node of more than G bodies packs its
+// children of at most G bodies into groups, consecutive siblings together
+// while the sum stays within G; a system root of at most G bodies is a group
+// of its own.  Every body lands in exactly one group and a group's bodies lie
+// inside one node, so groups are compact in space where runs of G
+// consecutive Morton keys are not (such a run can straddle two distant
+// cells).  Pass 1 counts each node's groups, pass 2 writes them at the
+// scanned offsets, so the table comes out in preorder of the packing nodes.
+// A node's body count is first(skip) - first, with n past the last node
+// (the next system's root starts at that system's first body).
+template <bool EMIT>
+__global__ __launch_bounds__(kBlock) void k_tree_groups(int64_t total, int64_t n, const RefNode* __restrict__ rn,
+                                                        const int32_t* __restrict__ body_sys, int G,
+                                                        uint32_t* __restrict__ cnt, Group* __restrict__ out) {
+  const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (id >= total) return;
+  auto count_of = [&](int64_t k, int32_t* skip_out) {
+    const int32_t sk = rn[k].skip;
+    *skip_out = sk;
+    return (int32_t)((sk < total ? (int64_t)rn[sk].first : n) - rn[k].first);
+  };
+  int32_t end = 0;
+  const int32_t count = count_of(id, &end);
+  uint32_t k = 0;
+  const uint32_t base = EMIT ? cnt[id] : 0u;
+  auto emit = [&](int32_t first, int32_t c) {
+    if (EMIT) out[base + k] = Group{first, c, body_sys[first], 0};
+    ++k;
+  };
+  if (count > G) {
+    int32_t run_first = 0, run = 0, nxt = 0;
+    for (int32_t c = (int32_t)id + 1, it = 0; c < end && it < 8; c = nxt, ++it) {
+      const int32_t cc = count_of(c, &nxt);
+      if (cc > G) {                                        // packs its own children
+        if (run > 0) emit(run_first, run);
+        run = 0;
+        continue;
+      }
+      if (run > 0 && run + cc > G) {
+        emit(run_first, run);
+        run = 0;
+      }
+      if (run == 0) run_first = rn[c].first;
+      run += cc;
+    }
+    if (run > 0) emit(run_first, run);
+  } else if (rn[id].depth == 0) {
+    emit(rn[id].first, count);
+  }
+  if (!EMIT) cnt[id] = k;
 }
 
 // Opening half kick and drift (step.h:294-325), each system with its dt.
@@ -806,6 +867,14 @@ void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, con
   else if (count) NBX_RW(false, true);
   else NBX_RW(false, false);
 #undef NBX_RW
+}
+void launch_tree_groups(int64_t total, int64_t n, const RefNode* rn, const int32_t* body_sys, int G,
+                        uint32_t* cnt, Group* out, uint32_t* ngroups, uint32_t* scratch, hipStream_t s) {
+  if (total <= 0) return;
+  const unsigned nb = (unsigned)div_up(total, kBlock);
+  hipLaunchKernelGGL(k_tree_groups<false>, dim3(nb), dim3(kBlock), 0, s, total, n, rn, body_sys, G, cnt, out);
+  scan_u32(cnt, cnt, total, ngroups, scratch, s);
+  hipLaunchKernelGGL(k_tree_groups<true>, dim3(nb), dim3(kBlock), 0, s, total, n, rn, body_sys, G, cnt, out);
 }
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
                        const SysArrays& sy, hipStream_t s) {

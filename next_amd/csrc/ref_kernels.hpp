@@ -49,6 +49,12 @@ void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const u
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
                  hipStream_t s);
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, hipStream_t s);
+// Walk groups of at most G bodies packed from the tree (k_tree_groups):
+// cnt (total words) holds the per-node counts and offsets, out needs room
+// for n groups, *ngroups (device) receives the count; scratch as for
+// scan_u32 over total elements.
+void launch_tree_groups(int64_t total, int64_t n, const RefNode* rn, const int32_t* body_sys, int G,
+                        uint32_t* cnt, Group* out, uint32_t* ngroups, uint32_t* scratch, hipStream_t s);
 void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
                  const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
                  bool count, hipStream_t s);

@@ -25,6 +25,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
            "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
+           "nbx_debug_groups",
            "nbx_potential", "nbx_direct", "nbx_energy", "nbx_energy_systems", "nbx_upload_batch", "nbx_add_system",
            "nbx_download_system", "nbx_remove_system", "nbx_num_systems", "nbx_system_size", "nbx_server_create",
            "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats"]
@@ -101,6 +102,7 @@ def lib():
     L.nbx_debug_keys.argtypes = [VP, f64p, u64p, u64p]
     L.nbx_debug_tree_size.argtypes = [VP, C.POINTER(C.c_int64)]
     L.nbx_debug_tree.argtypes = [VP, i32p, i32p, i32p, i32p, f64p, f64p, f64p, f64p, i32p]
+    L.nbx_debug_groups.argtypes = [VP, C.c_int64, i32p, i32p, i32p, C.POINTER(C.c_int64)]
     L.nbx_potential.argtypes = [VP, f64p]
     L.nbx_direct.argtypes = [VP, VP, VP, VP, VP]
     L.nbx_energy.argtypes = [VP, C.c_int, f64p]
@@ -293,6 +295,17 @@ class Engine:
         out = {key: v[:k] for key, v in t.items() if key != "perm"}
         out["perm"] = t["perm"][: self.n]
         return out
+
+
+    def debug_groups(self):
+        """Walk groups (first body in leaf order, count, system) of the current positions."""
+        k = C.c_int64()
+        z = np.zeros(1, np.int32)
+        self._check(lib().nbx_debug_groups(self._h, 0, z, z, z, C.byref(k)))
+        n = k.value
+        first, count, sys_ = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+        self._check(lib().nbx_debug_groups(self._h, n, first, count, sys_, C.byref(k)))
+        return first[:n], count[:n], sys_[:n]
 
 
 class Server:
