@@ -425,6 +425,41 @@ __device__ __forceinline__ int64_t xcd_block(int nb) {
   return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
+// Correctly rounded sqrt(x) for normal x >= 2^-767: the v_rsq_f64 estimate
+// refined by Goldschmidt steps and two residual corrections.  This is the
+// compiler's own expansion of sqrt without its range scaling and special-
+// value selects, which are no-ops for the walk's arguments (r^2 + 1e-20 and
+// r^2 + eps^2 with eps >= 1e-4: positive, finite, far above 2^-767), so the
+// results are bit for bit those of sqrt.
+__device__ __forceinline__ double sqrt_rn(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = 0.5 * y;
+  const double r = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
+// Correctly rounded 1/y for y in [2^-1000, 2^1000]: v_rcp_f64 refined by two
+// Newton steps, then the quotient residual correction.  The compiler's
+// division adds operand scaling (v_div_scale) and a special-value fixup
+// (v_div_fixup) that leave such operands unchanged, so the result is 1.0/y
+// bit for bit.  The walk's divisors are 1 + 10 dist >= 1 and
+// sqrt(r^2 + eps^2) >= 1e-4, both finite.
+__device__ __forceinline__ double rcp_rn(double y) {
+  double r = __builtin_amdgcn_rcp(y);
+  double e = __builtin_fma(-y, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-y, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-y, r, 1.0);
+  return __builtin_fma(e, r, r);
+}
+
 // Softening floor of a dark-matter target (octree.h:146-148), out of line:
 // expanded in place, the FP64 pow raises the walk's register count by about
 // half for every target.
@@ -517,12 +552,12 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
         const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
         const double m = nd->m;
         const double r2 = dx * dx + dy * dy + dz * dz;
-        const double dist = sqrt(r2 + 1e-20);
-        double eps = nd->esum * (1.0 / (1.0 + dist * 10.0));  // softening.h:26-36
+        const double dist = sqrt_rn(r2 + 1e-20);
+        double eps = nd->esum * rcp_rn(1.0 + dist * 10.0);   // softening.h:26-36
         if (eps < 1e-4) eps = 1e-4;
         if (pt == 1) eps = dm_softening(eps, nd->size, m, pm);   // octree.h:146-148
         const double r2s = r2 + eps * eps;
-        const double dinv = 1.0 / sqrt(r2s);
+        const double dinv = rcp_rn(sqrt_rn(r2s));
         const double inv3 = dinv * dinv * dinv;
         const double fac = m * inv3;                        // G = 1: (1.0 * m) * inv3
         a0 = a0 + dx * fac;
