@@ -729,28 +729,41 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift(int64_t n, StateArrays st
   st.z[i] = st.z[i] + vz * dt;
 }
 
-// Max |v|^2 per system (adaptive.h:31-37).  A wave whose bodies belong to one
-// system makes one atomic; a wave spanning systems, one per lane.
+// Max |v|^2 per system (adaptive.h:31-37).  Each thread takes kMaxv2Items
+// bodies (block-strided), a block whose bodies all belong to one system
+// reduces in registers and LDS and makes one atomic; otherwise every body
+// makes its own.
+constexpr int kMaxv2Items = 4;
 __global__ __launch_bounds__(kBlock) void k_maxv2(int64_t n, StateArrays st, SysArrays sy) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  __shared__ double red[kBlock / kWave];
+  const int64_t base = (int64_t)blockIdx.x * kBlock * kMaxv2Items;
+  const int64_t last = base + (int64_t)kBlock * kMaxv2Items - 1 < n ? base + (int64_t)kBlock * kMaxv2Items - 1 : n - 1;
+  const int32_t s0 = st.sys[base];
+  const bool one = st.sys[last] == s0;                    // sorted by system within the block's range
   double v2 = 0.0;
-  int32_t s = -1;
-  if (i < n) {
-    v2 = st.vx[i] * st.vx[i] + st.vy[i] * st.vy[i] + st.vz[i] * st.vz[i];
-    if (!(v2 > 0.0)) v2 = 0.0;
-    s = st.sys[i];
+  for (int k = 0; k < kMaxv2Items; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+    if (i >= n) break;
+    double w = st.vx[i] * st.vx[i] + st.vy[i] * st.vy[i] + st.vz[i] * st.vz[i];
+    if (!(w > 0.0)) w = 0.0;
+    if (one) {
+      if (w > v2) v2 = w;
+    } else if (w > 0.0) {
+      atomicMax(&sy.v2max[st.sys[i]], (unsigned long long)__double_as_longlong(w));
+    }
   }
-  const int32_t s0 = __shfl(s, 0, kWave);
-  if (__ballot(s != s0 && s >= 0)) {
-    if (s >= 0 && v2 > 0.0) atomicMax(&sy.v2max[s], (unsigned long long)__double_as_longlong(v2));
-    return;
-  }
+  if (!one) return;                                        // block-uniform
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const double o = __shfl_xor(v2, off, kWave);
     if (o > v2) v2 = o;
   }
-  if (lane_id() == 0 && s0 >= 0 && v2 > 0.0) atomicMax(&sy.v2max[s0], (unsigned long long)__double_as_longlong(v2));
+  if (lane_id() == 0) red[threadIdx.x / kWave] = v2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / kWave; ++w) v2 = red[w] > v2 ? red[w] : v2;
+    if (v2 > 0.0) atomicMax(&sy.v2max[s0], (unsigned long long)__double_as_longlong(v2));
+  }
 }
 
 // computeAdaptiveDt (adaptive.h:38-51) for each system from its max |v|^2;
@@ -916,7 +929,9 @@ void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const
   if (n > 0) hipLaunchKernelGGL(k_kick_drift, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, ax, ay, az, sy);
 }
 void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_maxv2, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy);
+  if (n > 0)
+    hipLaunchKernelGGL(k_maxv2, dim3((unsigned)div_up(n, (int64_t)kBlock * kMaxv2Items)), dim3(kBlock), 0, s, n, st,
+                       sy);
 }
 void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, hipStream_t s) {
   hipLaunchKernelGGL(k_dt, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, base, dts, step);
