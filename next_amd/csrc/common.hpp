@@ -18,15 +18,16 @@ constexpr int kMaxDepthSlots = 64; // histogram slots for node depths
 // visit needs sit in the first 64 bytes; the quadrupole, needed only when a
 // cell is accepted, in the second.  128 bytes = one L2 line.
 struct alignas(128) RefNode {
+  // first 56 bytes: what the walk reads for every node it visits
   double cx, cy, cz, m;    // centre of mass, mass
-  double size;             // half-width: root half-width * 2^-depth
-  double esum;             // size*0.015 + cbrt(m)*0.002 (softening numerator)
+  double acc2, open2;      // (size/theta)^2 (1 +- 1e-12): accept beyond, open below
   int32_t skip;            // preorder index of the first node after this subtree
   int32_t body;            // leaf: sorted body index; internal: -1
   int32_t depth;
   int32_t first;           // first body (sorted index) of the subtree
+  double size;             // half-width: root half-width * 2^-depth
+  double esum;             // size*0.015 + cbrt(m)*0.002 (softening numerator)
   double qxx, qyy, qzz, qxy, qxz, qyz;
-  double pad[2];
 };
 static_assert(sizeof(RefNode) == 128, "RefNode layout");
 

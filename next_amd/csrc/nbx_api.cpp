@@ -325,7 +325,11 @@ nbx_status build_tree(nbx_ctx* c) {
   const int64_t m = (uint32_t)c->hsc->internal_nodes;
   c->total_nodes = total;
   c->max_depth = c->hsc->max_leaf_depth;
-  NBXCHK(ensure(c, c->nodes, (size_t)total * sizeof(RefNode)));
+  // The walk reads node records at 32-bit byte offsets, and one record past
+  // the last node (a successor it loads but never uses).
+  if (total >= (int64_t)1 << 25)
+    return fail(c, NBX_ERR_ARG, "the tree has " + std::to_string(total) + " nodes; at most 33554431 are supported");
+  NBXCHK(ensure(c, c->nodes, (size_t)(total + 1) * sizeof(RefNode)));
   const size_t mm = (size_t)std::max<int64_t>(m, 1) * 4;
   for (DBuf* b : {&c->st_key, &c->st_val, &c->en_key, &c->en_val, &c->st_key2, &c->st_val2, &c->en_key2,
                   &c->en_val2, &c->level_list})
@@ -363,7 +367,7 @@ nbx_status build_tree(nbx_ctx* c) {
   } else {
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
-        launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, s);
+        launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, c->cfg.theta, s);
   }
   record(c, EV_UPWARD);
   c->tree_valid = true;

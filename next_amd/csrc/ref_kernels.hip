@@ -375,9 +375,10 @@ __global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const uint32_t* __re
 }
 
 // Mass, centre of mass and one-level softened quadrupole of the internal
-// nodes of one depth from their children (octree.h:104-126).
+// nodes of one depth from their children (octree.h:104-126), and the
+// squared-distance thresholds of the walk's opening test for this theta.
 __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt,
-                                                   RefNode* __restrict__ nodes) {
+                                                   RefNode* __restrict__ nodes, double inv_theta) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= cnt) return;
   const int32_t id = level_list[beg + k];
@@ -415,6 +416,10 @@ __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ l
   p.m = M; p.cx = cx; p.cy = cy; p.cz = cz;
   p.esum = size * 0.015 + cbrt(M) * 0.002;
   p.qxx = qxx; p.qyy = qyy; p.qzz = qzz; p.qxy = qxy; p.qxz = qxz; p.qyz = qyz;
+  const double q = size * inv_theta;
+  const double q2 = q * q;
+  p.acc2 = q2 * (1.0 + 1e-12);
+  p.open2 = q2 * (1.0 - 1e-12);
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks
@@ -470,15 +475,17 @@ __device__ __noinline__ double dm_softening(double eps, double size, double m, d
 
 // The per-visit fields of a node, held in scalar registers.
 struct NodeHead {
-  double cx, cy, cz, m, size;
+  double cx, cy, cz, m, acc2, open2;
   int32_t skip, body;
 };
 
-__device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes, int32_t k) {
-  const RefNode* p = nodes + k;
+// k < 2^25, checked on the host: the byte offset fits 32 bits and becomes the
+// scalar load's offset register.
+__device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes, uint32_t k) {
+  const RefNode* p = reinterpret_cast<const RefNode*>(reinterpret_cast<const char*>(nodes) + (k << 7));
   NodeHead h;
   h.cx = p->cx; h.cy = p->cy; h.cz = p->cz; h.m = p->m;
-  h.size = p->size; h.skip = p->skip; h.body = p->body;
+  h.acc2 = p->acc2; h.open2 = p->open2; h.skip = p->skip; h.body = p->body;
   return h;
 }
 
@@ -533,7 +540,6 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   int pt = 0;
   if (valid) { px = st.x[i]; py = st.y[i]; pz = st.z[i]; pm = st.m[i]; pt = st.type[i]; }
   const int32_t me = valid ? (int32_t)i : -2;
-  const double inv_theta = 1.0 / theta;
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   uint32_t npp = 0, npc = 0, nop = 0, nit = 0, nev = 0;
   int head = 0, len = 0;                                   // this lane's ring: [head, head + len)
@@ -589,9 +595,11 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   NodeHead h = load_head(nodes, root);
   while (cur < total) {
     // successors, loaded while this node is processed
+    // (the node array has one spare record after the last node, so both
+    // successors, at most total, are readable)
     const int32_t sk = h.skip > cur ? h.skip : cur + 1;   // skip > cur by construction
-    const NodeHead h1 = load_head(nodes, cur + 1 < total ? cur + 1 : cur);
-    const NodeHead h2 = load_head(nodes, sk < total ? sk : cur);
+    const NodeHead h1 = load_head(nodes, (uint32_t)cur + 1u);
+    const NodeHead h2 = load_head(nodes, (uint32_t)sk);
     bool open = false;
     if (next == cur) {
       if (h.m == 0.0 || h.body == me) {                    // octree.h:134-135
@@ -602,11 +610,9 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
         if (!leaf) {
           const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
           const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
-          const double q = h.size * inv_theta;
-          const double q2 = q * q;
-          if (r2e > q2 * (1.0 + 1e-12)) accept = true;
-          else if (r2e < q2 * (1.0 - 1e-12)) accept = false;
-          else accept = (h.size / sqrt(r2e)) < theta;
+          if (r2e > h.acc2) accept = true;
+          else if (r2e < h.open2) accept = false;
+          else accept = (nodes[cur].size / sqrt(r2e)) < theta;
         }
         if (accept) {
           mine[((head + len) & (kWalkList - 1)) * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
@@ -899,8 +905,10 @@ void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefN
                  hipStream_t s) {
   if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, st_val, en_val, nodes, level_list);
 }
-void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, hipStream_t s) {
-  if (cnt > 0) hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes);
+void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, double theta,
+                   hipStream_t s) {
+  if (cnt > 0)
+    hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes, 1.0 / theta);
 }
 void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
                  const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,

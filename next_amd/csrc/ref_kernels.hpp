@@ -48,7 +48,8 @@ void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const u
                  uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, hipStream_t s);
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
                  hipStream_t s);
-void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, hipStream_t s);
+void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, double theta,
+                   hipStream_t s);
 // Walk groups of at most G bodies packed from the tree (k_tree_groups):
 // cnt (total words) holds the per-node counts and offsets, out needs room
 // for n groups, *ngroups (device) receives the count; scratch as for
