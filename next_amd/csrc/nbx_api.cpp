@@ -230,9 +230,10 @@ int sys_bits(int nsys) {
   return b;
 }
 
-// Tree of the current positions: per-system roots, keys, sort, gather,
-// layout, multipoles.
-nbx_status build_tree(nbx_ctx* c) {
+// First half of a tree build (launches only): bounding boxes, keys, sort,
+// gather of the state into key order (flips c->cur), lcp, node counts and
+// their scan, and the copy of the device scalars to the host mirror.
+nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   const int64_t n = c->n;
   const int S = (int)c->sys_n.size();
   hipStream_t s = c->stream;
@@ -272,7 +273,23 @@ nbx_status build_tree(nbx_ctx* c) {
                    P<uint32_t>(c->lvl_part), c->sc, s);
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
   scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
-  NBXCHK(sync_scalars(c));
+  HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+  *skey_out = skey;
+  return NBX_OK;
+}
+
+// Second half: wait for the scalars, then (rarely) the two-word key path,
+// node layout and multipoles.
+nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
+  const int64_t n = c->n;
+  const int S = (int)c->sys_n.size();
+  hipStream_t s = c->stream;
+  SysArrays sy = sysa(c);
+  StateArrays St = view(c, c->cur);
+  uint32_t* scr = P<uint32_t>(c->scratch);
+  uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
+  uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
+  HIPCHK(c, hipStreamSynchronize(s));
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
 
   const uint64_t* skey2 = nullptr;
@@ -374,6 +391,14 @@ nbx_status build_tree(nbx_ctx* c) {
   return NBX_OK;
 }
 
+// Tree of the current positions: per-system roots, keys, sort, gather,
+// layout, multipoles.
+nbx_status build_tree(nbx_ctx* c) {
+  const uint64_t* skey = nullptr;
+  NBXCHK(build_pre(c, &skey));
+  return build_post(c, skey);
+}
+
 nbx_status walk(nbx_ctx* c, bool kick) {
   const bool count = (c->prof & 2) != 0;
   if (count) launch_fill_scalars(c->sc, 8, c->stream);
@@ -469,8 +494,11 @@ nbx_status download_range(nbx_ctx* c, const void* dev, size_t esz, int64_t off, 
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   char* o = static_cast<char*>(out);
-  for (int64_t k = 0; k < cnt; ++k)
-    std::memcpy(o + ((int64_t)id[(size_t)k] - base) * esz, &tmp[(size_t)k * esz], esz);
+  for (int64_t k = 0; k < cnt; ++k) {
+    const int64_t at = (int64_t)id[(size_t)k] - base;
+    if (at < 0 || at >= cnt) return fail(c, NBX_ERR_STATE, "inconsistent body indices on the device");
+    std::memcpy(o + at * esz, &tmp[(size_t)k * esz], esz);
+  }
   return NBX_OK;
 }
 
