@@ -1,10 +1,13 @@
 // Device-wide exclusive scan and stable LSD radix sort for gfx950.
 //
-// Scan: reduce-then-scan in three launches (tile sums, one-workgroup scan of
-// the tile sums, tile-local scan plus carry).  No inter-workgroup waiting.
+// Scan: reduce-then-scan in two launches (tile sums; tile-local scan plus the
+// sum of the preceding tile sums, which every tile forms itself).  No
+// inter-workgroup waiting.
 //
-// Radix sort: 8-bit digits, three launches per digit (tile histograms, scan
-// of the digit-major histogram table, scatter).  The scatter ranks keys
+// Radix sort: 8-bit digits, three launches per digit (tile histograms, one
+// workgroup per digit scanning that digit's column over the tiles, scatter;
+// the scatter adds the exclusive prefix of the digit totals, which every tile
+// forms itself from 256 values).  The scatter ranks keys
 // stably inside a tile with wave64 ballots (8 ballots give each lane the set
 // of lanes holding the same digit), stages the tile in LDS in digit order and
 // writes it out in runs, so consecutive lanes store consecutive addresses.
@@ -29,24 +32,21 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __re
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_scan_bsums(uint32_t* __restrict__ bsum, int64_t nb,
-                                                           uint32_t* __restrict__ total) {
-  __shared__ uint32_t scr[kScanBlock / kWave + 1];
-  uint32_t carry = 0;
-  for (int64_t c = 0; c < nb; c += kScanBlock) {
-    int64_t i = c + threadIdx.x;
-    uint32_t v = i < nb ? bsum[i] : 0u;
-    uint32_t tot;
-    uint32_t ex = block_excl_scan_u32<kScanBlock>(v, scr, &tot);
-    if (i < nb) bsum[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
+// Sum of bsum[0, k) by one workgroup (every thread gets it).
+__device__ uint32_t block_prefix(const uint32_t* __restrict__ bsum, int64_t k, uint32_t* scr) {
+  uint32_t s = 0;
+  for (int64_t i = threadIdx.x; i < k; i += kScanBlock) s += bsum[i];
+  uint32_t tot;
+  (void)block_excl_scan_u32<kScanBlock>(s, scr, &tot);
+  return tot;
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* in, uint32_t* out, int64_t n,
-                                                          const uint32_t* __restrict__ bsum) {
+                                                          const uint32_t* __restrict__ bsum, int64_t nb,
+                                                          uint32_t* __restrict__ total) {
   __shared__ uint32_t scr[kScanBlock / kWave + 1];
+  const uint32_t carry = block_prefix(bsum, blockIdx.x, scr);
+  if (total && blockIdx.x == nb - 1 && threadIdx.x == 0) *total = carry + bsum[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* in, ui
     s += v[j];
   }
   uint32_t tot;
-  uint32_t run = bsum[blockIdx.x] + block_excl_scan_u32<kScanBlock>(s, scr, &tot);
+  uint32_t run = carry + block_excl_scan_u32<kScanBlock>(s, scr, &tot);
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     if (base + j < n) out[base + j] = run;
@@ -85,12 +85,31 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const K* __restrict__
   hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = c;
 }
 
+// Digit d's column of the tile histograms (hist[d * ntiles + t]) becomes its
+// exclusive prefix over the tiles; coltot[d] receives the column total.
+__global__ __launch_bounds__(kSortBlock) void k_radix_colscan(uint32_t* __restrict__ hist, int64_t ntiles,
+                                                              uint32_t* __restrict__ coltot) {
+  __shared__ uint32_t scr[kSortBlock / kWave + 1];
+  uint32_t* col = hist + (int64_t)blockIdx.x * ntiles;
+  uint32_t carry = 0;
+  for (int64_t c = 0; c < ntiles; c += kSortBlock) {
+    const int64_t i = c + threadIdx.x;
+    const uint32_t v = i < ntiles ? col[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan_u32<kSortBlock>(v, scr, &tot);
+    if (i < ntiles) col[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) coltot[blockIdx.x] = carry;
+}
+
 template <typename K>
 __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               K* __restrict__ kout, uint32_t* __restrict__ vout,
                                                               int64_t n, int shift,
-                                                              const uint32_t* __restrict__ off, int64_t ntiles) {
+                                                              const uint32_t* __restrict__ off, int64_t ntiles,
+                                                              const uint32_t* __restrict__ coltot) {
   constexpr int W = kSortBlock / kWave;
   __shared__ K sk[kSortTile];
   __shared__ uint32_t sv[kSortTile];
@@ -140,7 +159,8 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
     uint32_t tot;
     uint32_t ds = block_excl_scan_u32<kSortBlock>(run, scr, &tot);
     dstart[t] = ds;
-    gbase[t] = off[(int64_t)t * ntiles + blockIdx.x];
+    const uint32_t dbase = block_excl_scan_u32<kSortBlock>(coltot[t], scr, &tot);   // keys of smaller digits
+    gbase[t] = dbase + off[(int64_t)t * ntiles + blockIdx.x];
   }
   __syncthreads();
 #pragma unroll
@@ -175,7 +195,7 @@ bool radix_sort_impl(K* k0, uint32_t* v0, K* k1, uint32_t* v1, int64_t n, int bi
   if (n <= 0) return false;
   const int64_t ntiles = div_up(n, kSortTile);
   uint32_t* hist = scratch;
-  uint32_t* scan_scr = scratch + kSortBins * ntiles;
+  uint32_t* coltot = scratch + kSortBins * ntiles;
   const int passes = (int)div_up(bits, 8);
   K* ki = k0; K* ko = k1;
   uint32_t* vi = v0; uint32_t* vo = v1;
@@ -183,9 +203,9 @@ bool radix_sort_impl(K* k0, uint32_t* v0, K* k1, uint32_t* v1, int64_t n, int bi
     const int shift = 8 * p;
     hipLaunchKernelGGL(k_radix_hist<K>, dim3((unsigned)ntiles), dim3(kSortBlock), 0, s, ki, n, shift, hist,
                        ntiles);
-    scan_u32(hist, hist, kSortBins * ntiles, nullptr, scan_scr, s);
+    hipLaunchKernelGGL(k_radix_colscan, dim3(kSortBins), dim3(kSortBlock), 0, s, hist, ntiles, coltot);
     hipLaunchKernelGGL(k_radix_scatter<K>, dim3((unsigned)ntiles), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n,
-                       shift, hist, ntiles);
+                       shift, hist, ntiles, coltot);
     K* tk = ki; ki = ko; ko = tk;
     uint32_t* tv = vi; vi = vo; vo = tv;
   }
@@ -201,8 +221,7 @@ void scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uin
   }
   const int64_t nb = div_up(n, kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanBlock), 0, s, in, n, scratch);
-  hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kScanBlock), 0, s, scratch, nb, total);
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanBlock), 0, s, in, out, n, scratch);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanBlock), 0, s, in, out, n, scratch, nb, total);
 }
 
 bool radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int bits,

@@ -30,7 +30,7 @@ void scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uin
 // Words of scratch a radix sort of n elements needs.
 inline size_t sort_scratch_words(int64_t n) {
   int64_t tiles = div_up(n, kSortTile);
-  return (size_t)(kSortBins * tiles) + scan_scratch_words(kSortBins * tiles);
+  return (size_t)(kSortBins * tiles) + kSortBins;
 }
 
 // Stable LSD radix sort of (key, value) pairs over bits [0, bits) in 8-bit
