@@ -81,12 +81,26 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
   const int32_t id = level_list[beg + k];
   const int32_t end = rn[id].skip;
   const int32_t first = rn[id].first;
-  double M = 0, sx = 0, sy = 0, sz = 0;
+  // One walk of the child chain (at most 8 children) records the child
+  // indices; -1 marks an empty slot.
+  int32_t chv[8];
   int32_t c = id + 1;
-  for (int it = 0; it < 8 && c < end; ++it, c = pn[c].skip) {
-    const double cm = aux[c].m;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    chv[j] = -1;
+    if (c < end) {
+      chv[j] = c;
+      c = pn[c].skip;
+    }
+  }
+  double M = 0, sx = 0, sy = 0, sz = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (chv[j] < 0) continue;
+    const int32_t k = chv[j];
+    const double cm = aux[k].m;
     M += cm;
-    sx += pn[c].cx * cm; sy += pn[c].cy * cm; sz += pn[c].cz * cm;
+    sx += pn[k].cx * cm; sy += pn[k].cy * cm; sz += pn[k].cz * cm;
   }
   double cx, cy, cz;
   if (M > 0) { cx = sx / M; cy = sy / M; cz = sz / M; }
@@ -94,10 +108,12 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
   double q[6] = {0, 0, 0, 0, 0, 0};
   double bsum = 0;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  c = id + 1;
-  for (int it = 0; it < 8 && c < end; ++it, c = pn[c].skip) {
-    const PhysAux& a = aux[c];
-    const double rx = pn[c].cx - cx, ry = pn[c].cy - cy, rz = pn[c].cz - cz;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (chv[j] < 0) continue;
+    const int32_t k = chv[j];
+    const PhysAux& a = aux[k];
+    const double rx = pn[k].cx - cx, ry = pn[k].cy - cy, rz = pn[k].cz - cz;
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double mc = a.m;
     q[0] += a.q[0] + mc * (3 * rx * rx - r2);

@@ -384,26 +384,39 @@ __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ l
   const int32_t id = level_list[beg + k];
   const int32_t end = nodes[id].skip;
   const double size = nodes[id].size;
-  double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-  // At most 8 children; the bound also keeps a malformed skip chain finite.
+  // One walk of the child chain (at most 8; the bound also keeps a malformed
+  // chain finite) gathers each child's mass and centre into registers; a
+  // slot past the last child keeps mass 0, which both sums skip.
+  double cmv[8], cxv[8], cyv[8], czv[8];
   int32_t c = id + 1;
-  for (int it = 0; it < 8 && c < end; ++it, c = nodes[c].skip) {
-    const double cm = nodes[c].m;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    cmv[j] = 0.0; cxv[j] = 0.0; cyv[j] = 0.0; czv[j] = 0.0;
+    if (c < end) {
+      const RefNode& ch = nodes[c];
+      cmv[j] = ch.m; cxv[j] = ch.cx; cyv[j] = ch.cy; czv[j] = ch.cz;
+      c = ch.skip;
+    }
+  }
+  double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const double cm = cmv[j];
     if (cm == 0.0) continue;
     M = M + cm;
-    sx = sx + nodes[c].cx * cm;
-    sy = sy + nodes[c].cy * cm;
-    sz = sz + nodes[c].cz * cm;
+    sx = sx + cxv[j] * cm;
+    sy = sy + cyv[j] * cm;
+    sz = sz + czv[j] * cm;
   }
   double cx = sx, cy = sy, cz = sz;
   if (M > 0.0) { cx = sx / M; cy = sy / M; cz = sz / M; }
   double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
   const double soft = size * size * 0.01;
-  c = id + 1;
-  for (int it = 0; it < 8 && c < end; ++it, c = nodes[c].skip) {
-    const double mc = nodes[c].m;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const double mc = cmv[j];
     if (mc == 0.0) continue;
-    const double rx = nodes[c].cx - cx, ry = nodes[c].cy - cy, rz = nodes[c].cz - cz;
+    const double rx = cxv[j] - cx, ry = cyv[j] - cy, rz = czv[j] - cz;
     const double r2 = rx * rx + ry * ry + rz * rz + soft;
     qxx = qxx + mc * (3.0 * rx * rx - r2);
     qyy = qyy + mc * (3.0 * ry * ry - r2);
