@@ -87,14 +87,22 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(StateArrays st, const B
 }
 
 // Root centre and half-width of each system from its blocks' partials
-// (step.h:272-279); min and max are exact, so the order does not matter.
-__global__ __launch_bounds__(kBlock) void k_bbox_final(const double* __restrict__ part,
-                                                       const int32_t* __restrict__ block_off, SysArrays sy) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
+// (step.h:272-279), one wave per system; min and max are exact, so the
+// order does not matter.
+__global__ __launch_bounds__(kWave) void k_bbox_final(const double* __restrict__ part,
+                                                      const int32_t* __restrict__ block_off, SysArrays sy) {
+  const int s = blockIdx.x;
   if (s >= sy.nsys) return;
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
-  for (int b = block_off[s]; b < block_off[s + 1]; ++b)
+  for (int b = block_off[s] + (int)threadIdx.x; b < block_off[s + 1]; b += kWave)
     for (int k = 0; k < 3; ++k) { lo_upd(lo[k], part[6 * b + k]); hi_upd(hi[k], part[6 * b + 3 + k]); }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int k = 0; k < 3; ++k) {
+      lo_upd(lo[k], __shfl_xor(lo[k], off, kWave));
+      hi_upd(hi[k], __shfl_xor(hi[k], off, kWave));
+    }
+  if (threadIdx.x != 0) return;
   const double cx = (lo[0] + hi[0]) * 0.5;
   const double cy = (lo[1] + hi[1]) * 0.5;
   const double cz = (lo[2] + hi[2]) * 0.5;
@@ -861,7 +869,7 @@ void launch_set_dt(const SysArrays& sy, double dt, hipStream_t s) {
 void launch_bbox(const StateArrays& st, const BoxBlock* blocks, int nblocks, const int32_t* block_off, double* part,
                  const SysArrays& sy, hipStream_t s) {
   if (nblocks > 0) hipLaunchKernelGGL(k_bbox_partial, dim3(nblocks), dim3(kBlock), 0, s, st, blocks, part);
-  hipLaunchKernelGGL(k_bbox_final, dim3(sys_grid(sy)), dim3(kBlock), 0, s, part, block_off, sy);
+  hipLaunchKernelGGL(k_bbox_final, dim3((unsigned)std::max(sy.nsys, 1)), dim3(kWave), 0, s, part, block_off, sy);
 }
 void launch_keys(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint32_t* idx,
                  DevScalars* scw, hipStream_t s) {

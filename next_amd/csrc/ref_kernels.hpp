@@ -17,7 +17,7 @@ struct StateArrays {
 };
 
 constexpr int kBlock = 256;
-constexpr int kBoxSpan = 8192;      // bodies per block of the bounding-box reduction
+constexpr int kBoxSpan = 2048;      // bodies per block of the bounding-box reduction
 // Row of per-block partial counts written by the lcp kernel.
 constexpr int kPartMaxDepth = kMaxDepthSlots;      // column: deepest leaf
 constexpr int kPartEqual = kMaxDepthSlots + 1;     // column: equal-key pairs
