@@ -65,7 +65,8 @@ struct nbx_ctx {
   DBuf s_root, s_dt, s_v2, s_soff, s_sroot, s_send;
   DBuf groups, boxes, box_off, box_part;
   int64_t ngroups = 0;
-  DBuf tgroups, tgcnt;        // physics mode: walk groups from the tree, per-node counts
+  DBuf tgroups;               // physics mode: walk groups packed from the tree
+  DBuf gmarks;                // physics mode: per-body group counts and marks
   int64_t ntgroups = 0;
   int nboxes = 0;
   DevScalars* sc = nullptr;   // device
@@ -374,15 +375,17 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   if (c->physics) {
     NBXCHK(ensure(c, c->pnodes, (size_t)total * sizeof(PhysNode)));
     NBXCHK(ensure(c, c->paux, (size_t)total * kPhysAuxBytes));
-    launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), St, P<PhysNode>(c->pnodes), c->paux.p, s);
+    NBXCHK(ensure(c, c->tgroups, (size_t)n * sizeof(Group)));
+    NBXCHK(ensure(c, c->gmarks, (size_t)n * 8));
+    int32_t* gcnt = P<int32_t>(c->gmarks);
+    uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
+    launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
+    launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), St, P<PhysNode>(c->pnodes), gcnt, gmark, s);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
         launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<PhysNode>(c->pnodes),
-                           c->paux.p, n, total, c->cfg.theta, s);
-    NBXCHK(ensure(c, c->tgroups, (size_t)n * sizeof(Group)));
-    NBXCHK(ensure(c, c->tgcnt, (size_t)total * 4));
-    launch_tree_groups(total, n, nodes, St.sys, c->cfg.group_size, P<uint32_t>(c->tgcnt), P<Group>(c->tgroups),
-                       &c->sc->tree_groups, scr, s);
+                           c->paux.p, St, n, total, c->cfg.theta, c->cfg.group_size, gcnt, gmark, s);
+    launch_group_compact(n, gcnt, gmark, St.sys, P<Group>(c->tgroups), &c->sc->tree_groups, scr, s);
     NBXCHK(sync_scalars(c));
     c->ntgroups = c->hsc->tree_groups;
   } else {
@@ -607,8 +610,7 @@ void nbx_destroy(nbx_ctx* c) {
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
                   &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout, &c->s_root, &c->s_dt, &c->s_v2,
-                  &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part, &c->tgroups,
-                  &c->tgcnt})
+                  &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part, &c->tgroups, &c->gmarks})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->sc) (void)hipFree(c->sc);

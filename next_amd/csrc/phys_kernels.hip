@@ -46,9 +46,20 @@ static_assert(sizeof(PhysAux) == kPhysAuxBytes, "PhysAux layout");
 
 inline unsigned grid_of(int64_t n, int b) { return (unsigned)div_up(n, b); }
 
+// A walk group of c bodies starts at sorted body f: marked in two zeroed
+// per-body arrays, compacted in body (Morton) order afterwards.
+__device__ __forceinline__ void mark_group(int32_t f, int32_t c, int32_t* gcnt, uint32_t* gmark) {
+  gcnt[f] = c;
+  gmark[f] = 1u;
+}
+
+// Leaf records (a leaf's FP64 moments are its body's: mass st.m, no
+// quadrupole, zero extent, so no auxiliary record is written).  A leaf that
+// is a system's root is that system's walk group.
 __global__ __launch_bounds__(kBlock) void k_phys_leaves(int64_t n, const int32_t* __restrict__ lcp,
                                                         const uint32_t* __restrict__ pre_off, StateArrays st,
-                                                        PhysNode* __restrict__ pn, PhysAux* __restrict__ aux) {
+                                                        PhysNode* __restrict__ pn, int32_t* __restrict__ gcnt,
+                                                        uint32_t* __restrict__ gmark) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int li = lcp[i];
@@ -63,19 +74,43 @@ __global__ __launch_bounds__(kBlock) void k_phys_leaves(int64_t n, const int32_t
   nd.first = (int32_t)i;
   nd.count = 1;
   pn[p] = nd;
-  PhysAux a;
-  a.m = st.m[i];
-  a.bmax = 0.0;
-  for (int k = 0; k < 6; ++k) a.q[k] = 0.0;
-  a.lo[0] = a.hi[0] = nd.cx; a.lo[1] = a.hi[1] = nd.cy; a.lo[2] = a.hi[2] = nd.cz;
-  a.pad[0] = a.pad[1] = 0.0;
-  aux[p] = a;
+  if (li < 0 && lp < 0) mark_group((int32_t)i, 1, gcnt, gmark);
 }
 
+// FP64 moments of child k: from its auxiliary record, or, for a leaf, from
+// its body.
+struct ChildMoments {
+  double m, bmax, q[6], lo[3], hi[3];
+};
+__device__ __forceinline__ void child_moments(int32_t k, const PhysNode* pn, const PhysAux* aux, const double* mass,
+                                              ChildMoments& o) {
+  if (pn[k].count == 1) {
+    o.m = mass[pn[k].first];
+    o.bmax = 0.0;
+    for (int j = 0; j < 6; ++j) o.q[j] = 0.0;
+    o.lo[0] = o.hi[0] = pn[k].cx; o.lo[1] = o.hi[1] = pn[k].cy; o.lo[2] = o.hi[2] = pn[k].cz;
+  } else {
+    const PhysAux& a = aux[k];
+    o.m = a.m;
+    o.bmax = a.bmax;
+    for (int j = 0; j < 6; ++j) o.q[j] = a.q[j];
+    for (int d = 0; d < 3; ++d) { o.lo[d] = a.lo[d]; o.hi[d] = a.hi[d]; }
+  }
+}
+
+// Internal nodes of one depth from their children (FP64), and the walk
+// groups this node packs: a node of more than G bodies packs its children of
+// at most G bodies into groups, consecutive siblings together while the sum
+// stays within G; a system root of at most G bodies is a group of its own.
+// Every body lands in exactly one group and a group's bodies lie inside one
+// node, so groups are compact in space where runs of G consecutive Morton
+// keys are not (such a run can straddle two distant cells).
 __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg,
                                                         int64_t cnt, const RefNode* __restrict__ rn,
                                                         PhysNode* __restrict__ pn, PhysAux* __restrict__ aux,
-                                                        int64_t n, int64_t total, double inv_theta) {
+                                                        const double* __restrict__ mass,
+                                                        int64_t n, int64_t total, double inv_theta, int G,
+                                                        int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= cnt) return;
   const int32_t id = level_list[beg + k];
@@ -97,10 +132,10 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     if (chv[j] < 0) continue;
-    const int32_t k = chv[j];
-    const double cm = aux[k].m;
+    const int32_t kc = chv[j];
+    const double cm = pn[kc].count == 1 ? mass[pn[kc].first] : aux[kc].m;
     M += cm;
-    sx += pn[k].cx * cm; sy += pn[k].cy * cm; sz += pn[k].cz * cm;
+    sx += pn[kc].cx * cm; sy += pn[kc].cy * cm; sz += pn[kc].cz * cm;
   }
   double cx, cy, cz;
   if (M > 0) { cx = sx / M; cy = sy / M; cz = sz / M; }
@@ -111,9 +146,10 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     if (chv[j] < 0) continue;
-    const int32_t k = chv[j];
-    const PhysAux& a = aux[k];
-    const double rx = pn[k].cx - cx, ry = pn[k].cy - cy, rz = pn[k].cz - cz;
+    const int32_t kc = chv[j];
+    ChildMoments a;
+    child_moments(kc, pn, aux, mass, a);
+    const double rx = pn[kc].cx - cx, ry = pn[kc].cy - cy, rz = pn[kc].cz - cz;
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double mc = a.m;
     q[0] += a.q[0] + mc * (3 * rx * rx - r2);
@@ -147,6 +183,26 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
   nd.skip = end;
   nd.first = first;
   pn[id] = nd;
+  // walk groups
+  if (nd.count > G) {
+    int32_t run_first = 0, run = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (chv[j] < 0) continue;
+      const int32_t cc = pn[chv[j]].count;
+      if (cc > G) {                                        // packs its own children
+        if (run > 0) mark_group(run_first, run, gcnt, gmark);
+        run = 0;
+        continue;
+      }
+      if (run > 0 && run + cc > G) { mark_group(run_first, run, gcnt, gmark); run = 0; }
+      if (run == 0) run_first = pn[chv[j]].first;
+      run += cc;
+    }
+    if (run > 0) mark_group(run_first, run, gcnt, gmark);
+  } else if (rn[id].depth == 0) {
+    mark_group(first, nd.count, gcnt, gmark);
+  }
   PhysAux a;
   a.m = M;
   a.bmax = bmax;
@@ -502,18 +558,38 @@ void walk_dispatch(unsigned nb, const Group* groups, int64_t ngroups, const Phys
 
 }  // namespace
 
+// Group table in body order from the marks (gmark holds the exclusive scan
+// of the marks by now; gcnt != 0 marks a group start).
+__global__ __launch_bounds__(kBlock) void k_group_compact(int64_t n, const int32_t* __restrict__ gcnt,
+                                                          const uint32_t* __restrict__ gidx,
+                                                          const int32_t* __restrict__ body_sys,
+                                                          Group* __restrict__ groups) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t c = gcnt[i];
+  if (c != 0) groups[gidx[i]] = Group{(int32_t)i, c, body_sys[i], 0};
+}
+
 void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st, PhysNode* pn,
-                        void* aux, hipStream_t s) {
+                        int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_phys_leaves, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, st, pn,
-                       reinterpret_cast<PhysAux*>(aux));
+    hipLaunchKernelGGL(k_phys_leaves, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, st, pn, gcnt,
+                       gmark);
 }
 
 void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
-                        void* aux, int64_t n, int64_t total, double theta, hipStream_t s) {
+                        void* aux, const StateArrays& st, int64_t n, int64_t total, double theta, int group_size,
+                        int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
   if (cnt > 0)
     hipLaunchKernelGGL(k_phys_upward, dim3(grid_of(cnt, kBlock)), dim3(kBlock), 0, s, level_list, beg, cnt, rn, pn,
-                       reinterpret_cast<PhysAux*>(aux), n, total, 1.0 / theta);
+                       reinterpret_cast<PhysAux*>(aux), st.m, n, total, 1.0 / theta, group_size, gcnt, gmark);
+}
+
+void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const int32_t* body_sys, Group* groups,
+                          uint32_t* ngroups, uint32_t* scratch, hipStream_t s) {
+  if (n <= 0) return;
+  scan_u32(gmark, gmark, n, ngroups, scratch, s);
+  hipLaunchKernelGGL(k_group_compact, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, gcnt, gmark, body_sys, groups);
 }
 
 void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,

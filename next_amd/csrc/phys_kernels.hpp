@@ -32,14 +32,23 @@ constexpr int kPhysMaxLeaf = 64;    // bucket size limit (one body per lane)
 constexpr int kPhysStack = 256;     // per-wave LDS stack of donated node ranges
 constexpr int kPhysAuxBytes = 128;  // FP64 moments per node (mass, bmax, Q, box)
 
-// Leaves: one PhysNode per body.
-// `aux`: kPhysAuxBytes per node of FP64 moments (mass, bmax, quadrupole,
-// bounding box of the bodies).
-void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st,
-                        PhysNode* pn, void* aux, hipStream_t s);
-// Internal nodes of one depth from their children (FP64 accumulation).
+// Walk groups are marked per first body in gcnt (body count) and gmark
+// (1), both zeroed beforehand, then compacted in body order.
+// Leaves: one PhysNode per body; a leaf that is a system root is a one-body
+// group.
+void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st, PhysNode* pn,
+                        int32_t* gcnt, uint32_t* gmark, hipStream_t s);
+// Internal nodes of one depth from their children (FP64 accumulation), and
+// the walk groups of at most group_size bodies each node packs (see
+// k_phys_upward).  `aux`: kPhysAuxBytes per node of FP64 moments (mass,
+// bmax, quadrupole, bounding box of the bodies), internal nodes only.
 void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
-                        void* aux, int64_t n, int64_t total, double theta, hipStream_t s);
+                        void* aux, const StateArrays& st, int64_t n, int64_t total, double theta, int group_size,
+                        int32_t* gcnt, uint32_t* gmark, hipStream_t s);
+// The group table from the marks (gmark is overwritten by its scan);
+// *ngroups (device) receives the count; scratch as for scan_u32 over n.
+void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const int32_t* body_sys, Group* groups,
+                          uint32_t* ngroups, uint32_t* scratch, hipStream_t s);
 // Group walk over the group table: accelerations (and potentials if
 // pot != null), optionally followed by the closing half kick.  fp64 selects
 // the FP64 evaluation.

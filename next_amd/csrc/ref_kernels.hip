@@ -64,6 +64,11 @@ __global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int wha
   if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; sc->n_eval = 0ull; }
 }
 
+__global__ __launch_bounds__(kBlock) void k_fill_u32(int64_t n, uint32_t* __restrict__ a, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_v2max(SysArrays sy) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s < sy.nsys) sy.v2max[s] = 0ull;
@@ -687,58 +692,6 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   }
 }
 
-// Walk groups from the tree.  A node of more than G bodies packs its
-// children of at most G bodies into groups, consecutive siblings together
-// while the sum stays within G; a system root of at most G bodies is a group
-// of its own.  Every body lands in exactly one group and a group's bodies lie
-// inside one node, so groups are compact in space where runs of G
-// consecutive Morton keys are not (such a run can straddle two distant
-// cells).  Pass 1 counts each node's groups, pass 2 writes them at the
-// scanned offsets, so the table comes out in preorder of the packing nodes.
-// A node's body count is first(skip) - first, with n past the last node
-// (the next system's root starts at that system's first body).
-template <bool EMIT>
-__global__ __launch_bounds__(kBlock) void k_tree_groups(int64_t total, int64_t n, const RefNode* __restrict__ rn,
-                                                        const int32_t* __restrict__ body_sys, int G,
-                                                        uint32_t* __restrict__ cnt, Group* __restrict__ out) {
-  const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (id >= total) return;
-  auto count_of = [&](int64_t k, int32_t* skip_out) {
-    const int32_t sk = rn[k].skip;
-    *skip_out = sk;
-    return (int32_t)((sk < total ? (int64_t)rn[sk].first : n) - rn[k].first);
-  };
-  int32_t end = 0;
-  const int32_t count = count_of(id, &end);
-  uint32_t k = 0;
-  const uint32_t base = EMIT ? cnt[id] : 0u;
-  auto emit = [&](int32_t first, int32_t c) {
-    if (EMIT) out[base + k] = Group{first, c, body_sys[first], 0};
-    ++k;
-  };
-  if (count > G) {
-    int32_t run_first = 0, run = 0, nxt = 0;
-    for (int32_t c = (int32_t)id + 1, it = 0; c < end && it < 8; c = nxt, ++it) {
-      const int32_t cc = count_of(c, &nxt);
-      if (cc > G) {                                        // packs its own children
-        if (run > 0) emit(run_first, run);
-        run = 0;
-        continue;
-      }
-      if (run > 0 && run + cc > G) {
-        emit(run_first, run);
-        run = 0;
-      }
-      if (run == 0) run_first = rn[c].first;
-      run += cc;
-    }
-    if (run > 0) emit(run_first, run);
-  } else if (rn[id].depth == 0) {
-    emit(rn[id].first, count);
-  }
-  if (!EMIT) cnt[id] = k;
-}
-
 // Opening half kick and drift (step.h:294-325), each system with its dt.
 __global__ __launch_bounds__(kBlock) void k_kick_drift(int64_t n, StateArrays st, const double* __restrict__ ax,
                                                        const double* __restrict__ ay, const double* __restrict__ az,
@@ -945,13 +898,8 @@ void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, con
   else NBX_RW(false, false);
 #undef NBX_RW
 }
-void launch_tree_groups(int64_t total, int64_t n, const RefNode* rn, const int32_t* body_sys, int G,
-                        uint32_t* cnt, Group* out, uint32_t* ngroups, uint32_t* scratch, hipStream_t s) {
-  if (total <= 0) return;
-  const unsigned nb = (unsigned)div_up(total, kBlock);
-  hipLaunchKernelGGL(k_tree_groups<false>, dim3(nb), dim3(kBlock), 0, s, total, n, rn, body_sys, G, cnt, out);
-  scan_u32(cnt, cnt, total, ngroups, scratch, s);
-  hipLaunchKernelGGL(k_tree_groups<true>, dim3(nb), dim3(kBlock), 0, s, total, n, rn, body_sys, G, cnt, out);
+void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, n, a, v);
 }
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
                        const SysArrays& sy, hipStream_t s) {

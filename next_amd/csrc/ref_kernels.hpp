@@ -50,12 +50,7 @@ void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefN
                  hipStream_t s);
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, double theta,
                    hipStream_t s);
-// Walk groups of at most G bodies packed from the tree (k_tree_groups):
-// cnt (total words) holds the per-node counts and offsets, out needs room
-// for n groups, *ngroups (device) receives the count; scratch as for
-// scan_u32 over total elements.
-void launch_tree_groups(int64_t total, int64_t n, const RefNode* rn, const int32_t* body_sys, int G,
-                        uint32_t* cnt, Group* out, uint32_t* ngroups, uint32_t* scratch, hipStream_t s);
+void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s);
 void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
                  const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
                  bool count, hipStream_t s);
