@@ -10,10 +10,13 @@ resident in HBM.
 
 Prints one JSON line: ms per KDK step, the walk kernel's roofline (FP64
 VALU; algorithmic flops 19 per particle-particle and 55 per particle-cell
-interaction, counted on the device), the per-phase split, and NEXT's own CPU
-code timed on this host on a bounded sample of the same workload.
+interaction, counted on the device; HBM traffic from the committed PMC
+summary), the per-phase split with the HBM rate of the streaming phases, a
+bounded sample of the batched-ensemble and serving workloads (physics mode),
+and NEXT's own CPU code timed on this host on a bounded sample of the
+headline workload.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extra] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -133,6 +136,27 @@ def serving(nreq: int, rate: float, seed: int = 5) -> dict:
                         "theta {0.4, 0.5, 0.7}, FP32/FP64 walk, dt 1e-3, eps 0.01, 200 GB budget"}
 
 
+# Algorithmic HBM bytes per body of the streaming phases (state: x y z vx vy
+# vz m as FP64, type id sys as int32; keys 8 B, sort values 4 B):
+#   kick + drift: read x..vz, a, sys (76 B), write x..vz (48 B);
+#   bbox + keys: bbox reads x y z (24 B); keys read x y z sys (28 B) and write
+#     key and index (12 B);
+#   sort + gather: 8 radix passes, each reading the keys for the histogram
+#     (8 B) and keys + values for the scatter (12 B) and writing them (12 B);
+#     then the gather reads the permutation (4 B) and the 68-byte state and
+#     writes it (68 B).
+PHASE_BYTES = {"kick_ms": 124, "keys_ms": 64, "sort_ms": 8 * 32 + 140}
+
+
+def phase_rooflines(n: int, tim: dict, k: int) -> dict:
+    out = {}
+    for key, per_body in PHASE_BYTES.items():
+        ms = tim[key] / k
+        gbs = per_body * n / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        out[key[:-3]] = {"bytes_per_body": per_body, "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def load_traffic() -> dict | None:
     path = os.path.join(ROOT, "profiles", "walk_traffic.json")
     if not os.path.exists(path):
@@ -234,6 +258,7 @@ def main() -> int:
         "phases_ms": {key: round(tim[key] / k, 4) for key in
                       ("kick_ms", "keys_ms", "sort_ms", "build_ms", "upward_ms", "walk_ms", "total_ms")},
         "tree": {"nodes": tim["nodes"], "depth": tim["depth"]},
+        "phases_hbm": phase_rooflines(args.n, tim, k),
     }
     log("headline: %.3f ms/step" % ms_per_step)
     if not args.no_extra:
