@@ -64,13 +64,37 @@ def build_library() -> str:
     return LIB
 
 
+HDF5_ROOT = os.environ.get("NBX_HDF5_ROOT", "/opt/conda")
+
+
+def hdf5_flags() -> list[str]:
+    """HDF5 snapshots in the CLI when the image has libhdf5.
+
+    The library's directory also holds an older libstdc++, so the binary
+    does not search it: bin/hdf5/ gets links to libhdf5 and its libz only,
+    and that is the binary's run path."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    so = os.path.join(lib, "libhdf5.so")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(so)):
+        return []
+    side = os.path.join(os.path.dirname(BIN), "hdf5")
+    os.makedirs(side, exist_ok=True)
+    for name in os.listdir(lib):
+        if name.startswith("libhdf5.so.") or name.startswith("libz.so."):
+            link = os.path.join(side, name)
+            if os.path.lexists(link):
+                os.remove(link)
+            os.symlink(os.path.join(lib, name), link)
+    return ["-DNBX_HDF5", "-I", inc, so, "-Wl,-rpath,$ORIGIN/hdf5", f"-Wl,-rpath-link,{side}"]
+
+
 def build_cli() -> str | None:
     srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp")) if os.path.isdir(HOST) else []
     if not srcs:
         return None
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", HOST, *srcs,
-           "-o", BIN + ".tmp", "-L", PKG, "-lnbx", "-Wl,-rpath,$ORIGIN/.."]
+           "-o", BIN + ".tmp", "-L", PKG, "-lnbx", "-Wl,-rpath,$ORIGIN/..", *hdf5_flags()]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"CLI build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

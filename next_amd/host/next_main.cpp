@@ -110,6 +110,16 @@ bool quit_requested() {
   return false;
 }
 
+bool save(const nextio::ParticleSet& ps, Format f, const std::string& path) {
+  if (f == Format::VTK) return nextio::SaveVTK(ps, path);
+  if (f == Format::VTU) return nextio::SaveVTU(ps, path);
+#ifdef NBX_HDF5
+  return nextio::SaveHDF5(ps, path);
+#else
+  return false;
+#endif
+}
+
 int die(nbx_ctx* ctx, nbx_status s, const char* what) {
   std::cerr << "next: " << what << " failed (" << (int)s << "): " << nbx_last_error(ctx) << "\n";
   nbx_destroy(ctx);
@@ -120,10 +130,12 @@ int die(nbx_ctx* ctx, nbx_status s, const char* what) {
 
 int main(int argc, char** argv) {
   Args args = parse(argc, argv);
+#ifndef NBX_HDF5
   if (args.format == Format::HDF5) {
-    std::cerr << "next: hdf5 snapshots are not part of this build; use vtk or vtu\n";
+    std::cerr << "next: this build has no libhdf5; use vtk or vtu\n";
     return 1;
   }
+#endif
   static const char* kBanner = R"(
  _  _ ________   _________
 | \ | |  ____\ \ / /__   __|
@@ -136,11 +148,10 @@ int main(int argc, char** argv) {
   std::cout << " Threads:   " << args.threads << std::endl;
   std::cout << " Precision: FP64" << std::endl;
 
-  nextio::ParticleSet ps = nextio::LoadText(args.input);
+  nextio::ParticleSet ps = nextio::LoadParticles(args.input);
   std::cout << " Particles: " << ps.size() << std::endl;
   if (!args.dump_ic.empty()) {
-    const bool ok = args.format == Format::VTK ? nextio::SaveVTK(ps, args.dump_ic) : nextio::SaveVTU(ps, args.dump_ic);
-    return ok ? 0 : 1;
+    return save(ps, args.format, args.dump_ic) ? 0 : 1;
   }
 
   nbx_config cfg;
@@ -179,8 +190,8 @@ int main(int argc, char** argv) {
       if (!args.no_dumps) {
         s = nbx_download(ctx, ps.x.data(), ps.y.data(), ps.z.data(), ps.vx.data(), ps.vy.data(), ps.vz.data());
         if (s != NBX_OK) return die(ctx, s, "download");
-        if (args.format == Format::VTK) { out += ".vtk"; nextio::SaveVTK(ps, out); }
-        else { out += ".vtu"; nextio::SaveVTU(ps, out); }
+        out += args.format == Format::VTK ? ".vtk" : args.format == Format::VTU ? ".vtu" : ".hdf5";
+        save(ps, args.format, out);
       }
       std::cout << "[Dump " << dump << "] t = " << sim_time << ", file: " << out << std::endl;
       next_dump += args.dump_interval;

@@ -10,6 +10,7 @@ change; the committed ``.npz`` files are what the tests read.
 from __future__ import annotations
 
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -91,6 +92,23 @@ def writer_case(name: str, p: ic.Particles):
     print(f"{name}: writers")
 
 
+H5 = os.path.join(ROOT, "oracle", "_ref", "ref_h5")
+
+
+def hdf5_case(name: str):
+    """NEXT's SaveHDF5 of the writer case (file written as dump_0.hdf5, the
+    name its XDMF sidecar records), and NEXT's loader reading that file back
+    (written out with SaveVTK)."""
+    with tempfile.TemporaryDirectory() as td:
+        subprocess.run([H5, "h5save", os.path.join(HERE, name + ".ic.txt"), "dump_0.hdf5"], check=True, cwd=td,
+                       stderr=subprocess.DEVNULL)
+        shutil.copyfile(os.path.join(td, "dump_0.hdf5"), os.path.join(HERE, name + ".hdf5"))
+        shutil.copyfile(os.path.join(td, "dump_0.xdmf"), os.path.join(HERE, name + ".xdmf"))
+        subprocess.run([H5, "h5load", os.path.join(HERE, name + ".hdf5"), os.path.join(HERE, name + ".h5load.vtk")],
+                       check=True, stderr=subprocess.DEVNULL)
+    print(f"{name}: hdf5")
+
+
 def hand_cases():
     """Small inputs that exercise single branches of the reference."""
     def mk(pos, m=None, types=None, vel=None):
@@ -140,7 +158,12 @@ def main():
     small = ic.plummer(5, seed=3, virial=None)
     small.type[1] = 1
     writer_case("writers_small", small)
+    if os.path.exists(H5):
+        hdf5_case("writers_small")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["hdf5"]:
+        hdf5_case("writers_small")
+    else:
+        main()
