@@ -84,13 +84,15 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
     samples of the distribution function, not rescaled to an exact virial
     ratio), N_i log-uniform in [1024, 65536] from seed i, theta = 0.5,
     dt = 1e-3, FP32 walk with FP64 state, all advanced by one set of launches
-    per step."""
+    per step; leaf and group size chosen by the autotuner first."""
     from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
 
     sizes = ic.ensemble_sizes(nsys)
     systems = [ic.plummer(n, seed=i, virial=None) for i, n in enumerate(sizes)]
+    cands = [(leaf, group) for leaf in (8, 16, 32) for group in (32, 64)]
     with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False) as e:
         e.upload_batch(systems)
+        best, trial_ms = e.autotune(cands, 1e-3, steps=2)   # leaves the state as it was
         e.run(1e-3, warmup)
         t0 = time.perf_counter()
         e.run(1e-3, steps)
@@ -98,6 +100,8 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
     total = int(sum(sizes))
     return {"systems": nsys, "bodies": total, "steps": steps, "ensemble_steps_per_s": steps / dt,
             "ms_per_step": dt * 1e3 / steps, "body_steps_per_s": total * steps / dt,
+            "autotuned": {"leaf_size": best[0], "group_size": best[1],
+                          "trial_ms_per_step": {f"{lf}/{g}": round(float(m), 3) for (lf, g), m in zip(cands, trial_ms)}},
             "workload": f"{nsys} Plummer spheres (DF samples), N log-uniform [1024, 65536], theta 0.5 (bmax criterion), "
                         "eps 0.01, dt 1e-3, FP32 walk / FP64 state"}
 

@@ -170,6 +170,15 @@ nbx_status nbx_energy(nbx_ctx* ctx, int source, double* out5);
 /* The same per system: out receives 5 values per system. */
 nbx_status nbx_energy_systems(nbx_ctx* ctx, int source, double* out);
 
+/* Autotuner (physics mode; no counterpart in NEXT).  Times `steps` fixed-dt
+ * steps of the current systems for each of the ncand candidate
+ * (leaf[k], group[k]) pairs, restoring the state after every trial, and
+ * keeps the fastest pair in the context's configuration.  ms_out (may be
+ * null, ncand entries) receives each candidate's ms per step.  The state
+ * the call leaves behind is the one it started from. */
+nbx_status nbx_autotune(nbx_ctx* ctx, const int32_t* leaf, const int32_t* group, int32_t ncand, double dt,
+                        int32_t steps, double* ms_out);
+
 /* Accumulated timings and counters. */
 nbx_status nbx_get_timing(nbx_ctx* ctx, nbx_timing* out);
 nbx_status nbx_reset_timing(nbx_ctx* ctx);

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -620,6 +621,66 @@ void nbx_destroy(nbx_ctx* c) {
 }
 
 const char* nbx_last_error(const nbx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, int32_t ncand, double dt,
+                        int32_t steps, double* ms_out) {
+  NBXCHK(check_ctx(c));
+  if (!c->physics) return fail(c, NBX_ERR_ARG, "the autotuner chooses physics-mode leaf and group sizes");
+  if (ncand <= 0 || !leaf || !group || steps <= 0 || !(dt > 0.0)) return fail(c, NBX_ERR_ARG, "bad candidates");
+  for (int32_t k = 0; k < ncand; ++k)
+    if (leaf[k] < 1 || leaf[k] > kPhysMaxLeaf || group[k] < 1 || group[k] > kPhysBlock)
+      return fail(c, NBX_ERR_ARG, "candidate out of range");
+  if (c->n == 0) return NBX_OK;
+  // The state every trial starts from (and that the call leaves behind).
+  const int cur0 = c->cur;
+  DBuf saved[kStateArrays];
+  auto release_all = [&]() { for (auto& b : saved) release(b); };
+  for (int k = 0; k < kStateArrays; ++k) {
+    const size_t bytes = (size_t)c->n * elem_bytes(k);
+    nbx_status st = ensure(c, saved[k], bytes);
+    if (st == NBX_OK && hipMemcpyAsync(saved[k].p, c->st[cur0][k].p, bytes, hipMemcpyDeviceToDevice, c->stream) !=
+                            hipSuccess)
+      st = fail(c, NBX_ERR_HIP, "state copy failed");
+    if (st != NBX_OK) { release_all(); return st; }
+  }
+  auto restore = [&]() -> nbx_status {
+    for (int k = 0; k < kStateArrays; ++k)
+      HIPCHK(c, hipMemcpyAsync(c->st[cur0][k].p, saved[k].p, (size_t)c->n * elem_bytes(k), hipMemcpyDeviceToDevice,
+                               c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = cur0;
+    c->acc_valid = false;
+    c->tree_valid = false;
+    return NBX_OK;
+  };
+  const nbx_config keep = c->cfg;
+  const nbx_timing tim0 = c->tim;                          // trials do not count as steps
+  int32_t best = -1;
+  double best_ms = 0.0;
+  for (int32_t k = 0; k < ncand; ++k) {
+    c->cfg.leaf_size = leaf[k];
+    c->cfg.group_size = group[k];
+    nbx_status st = nbx_run(c, dt, 1, nullptr);            // warm-up: buffers, first tree
+    const auto t0 = std::chrono::steady_clock::now();
+    if (st == NBX_OK) st = nbx_run(c, dt, steps, nullptr);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+    const nbx_status rs = restore();
+    if (st != NBX_OK || rs != NBX_OK) {
+      c->cfg = keep;
+      c->tim = tim0;
+      release_all();
+      return st != NBX_OK ? st : rs;
+    }
+    if (ms_out) ms_out[k] = ms;
+    if (best < 0 || ms < best_ms) { best = k; best_ms = ms; }
+  }
+  release_all();
+  c->tim = tim0;
+  c->cfg = keep;
+  c->cfg.leaf_size = leaf[best];
+  c->cfg.group_size = group[best];
+  return NBX_OK;
+}
 
 nbx_status nbx_set_theta(nbx_ctx* c, double theta) {
   if (!c || !(theta >= 0.0)) return NBX_ERR_ARG;

@@ -13,15 +13,16 @@
 // and bmax, a bound on the distance from the centre of mass to any body:
 //   bmax = max_c (|r_c| + bmax_c).
 //
-// Walk: one wave = one group of 64 consecutive (Morton-ordered) targets.
-// The wave walks the preorder tree with a decision shared by the whole
-// group: a node is accepted when the distance from its centre of mass to the
-// group's bounding box exceeds bmax/theta.  Node records are read at a
-// wave-uniform index (scalar loads).  Accepted cells and bucket bodies are
-// appended, relative to the group centre, to per-wave LDS lists; a full list
-// is evaluated by all 64 lanes at once (every lane busy, entries read as LDS
-// broadcasts).  Each flush's sum is formed in the evaluation precision and
-// added to FP64 totals.
+// Walk: one wave = one group of at most 64 targets packed from the tree
+// (consecutive sibling subtrees inside one node).  The wave walks the
+// preorder tree with a decision shared by the whole group: a node is
+// accepted when the distance from its centre of mass to the group's bounding
+// box exceeds bmax/theta.  Lanes walk disjoint ranges of the preorder array
+// and hand work to idle lanes (k_phys_walk).  Accepted cells and bucket
+// bodies are appended, relative to the group centre, to per-wave LDS lists;
+// a full list is evaluated by all the group's lanes at once (entries read as
+// LDS broadcasts).  Each flush's sum is formed in the evaluation precision
+// and added to FP64 totals.
 //
 // Acceleration on a target at distance vector d = com - x from a cell:
 //   a = m d / r^3 - Q d / r^5 + (5/2) (d^T Q d) d / r^7,  r^2 = |d|^2 + eps^2

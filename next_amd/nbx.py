@@ -25,7 +25,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
            "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
-           "nbx_debug_groups",
+           "nbx_debug_groups", "nbx_autotune",
            "nbx_potential", "nbx_direct", "nbx_energy", "nbx_energy_systems", "nbx_upload_batch", "nbx_add_system",
            "nbx_download_system", "nbx_remove_system", "nbx_num_systems", "nbx_system_size", "nbx_server_create",
            "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats"]
@@ -103,6 +103,7 @@ def lib():
     L.nbx_debug_tree_size.argtypes = [VP, C.POINTER(C.c_int64)]
     L.nbx_debug_tree.argtypes = [VP, i32p, i32p, i32p, i32p, f64p, f64p, f64p, f64p, i32p]
     L.nbx_debug_groups.argtypes = [VP, C.c_int64, i32p, i32p, i32p, C.POINTER(C.c_int64)]
+    L.nbx_autotune.argtypes = [VP, i32p, i32p, C.c_int32, C.c_double, C.c_int32, f64p]
     L.nbx_potential.argtypes = [VP, f64p]
     L.nbx_direct.argtypes = [VP, VP, VP, VP, VP]
     L.nbx_energy.argtypes = [VP, C.c_int, f64p]
@@ -296,6 +297,16 @@ class Engine:
         out["perm"] = t["perm"][: self.n]
         return out
 
+
+    def autotune(self, candidates, dt: float, steps: int = 2):
+        """Time each (leaf_size, group_size) candidate on the current systems
+        (physics mode), keep the fastest; returns (best, ms per step of each)."""
+        leaf = np.array([c[0] for c in candidates], np.int32)
+        group = np.array([c[1] for c in candidates], np.int32)
+        ms = np.zeros(len(candidates))
+        self._check(lib().nbx_autotune(self._h, leaf, group, len(candidates), dt, steps, ms))
+        best = int(np.argmin(ms))
+        return (int(leaf[best]), int(group[best])), ms
 
     def debug_groups(self):
         """Walk groups (first body in leaf order, count, system) of the current positions."""

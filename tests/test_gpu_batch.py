@@ -95,3 +95,24 @@ def test_many_small_systems():
             e.upload(sy[s])
             e.run(1e-3, 3)
             assert np.array_equal(g, e.download())
+
+
+def test_autotune_restores_state_and_applies_choice():
+    """nbx_autotune times candidate (leaf, group) sizes on the current
+    systems, leaves the state as it found it, and the steps that follow
+    equal a run made with the chosen sizes from the start."""
+    sy = [ic.plummer(n, seed=40 + i) for i, n in enumerate((3000, 1200, 6000))]
+    cands = [(8, 64), (16, 64), (16, 32), (32, 64)]
+    with make("physics", False) as e:
+        e.upload_batch(sy)
+        before = e.download()
+        best, ms = e.autotune(cands, 1e-3, steps=2)
+        assert np.array_equal(e.download(), before)
+        assert best in cands and len(ms) == len(cands) and np.all(ms > 0)
+        e.run(1e-3, 3)
+        after = e.download()
+    with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=best[0],
+                group_size=best[1], adaptive_dt=False) as e:
+        e.upload_batch(sy)
+        e.run(1e-3, 3)
+        assert np.array_equal(e.download(), after)
