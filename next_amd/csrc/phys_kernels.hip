@@ -218,6 +218,17 @@ struct alignas(4 * sizeof(T)) V4 {
   T x, y, z, w;
 };
 
+// Sum over the lanes that share a target slot (lanes slot + p * width).
+template <typename T>
+__device__ __forceinline__ void combine_parts(T& fx, T& fy, T& fz, T& fp, int width) {
+  for (int off = width; off < kWave; off <<= 1) {
+    fx += __shfl_xor(fx, off, kWave);
+    fy += __shfl_xor(fy, off, kWave);
+    fz += __shfl_xor(fz, off, kWave);
+    fp += __shfl_xor(fp, off, kWave);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T rsq(T x);
 template <>
@@ -260,9 +271,16 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   const int64_t gi = xcd_group(gridDim.x);
   if (gi >= ngroups) return;                               // whole wave past the end
   const Group g = groups[gi];
-  const int64_t i = g.first + lane;
-  const bool valid = lane < g.count;
-  const int64_t ii = valid ? i : g.first;
+  // A group of at most 32 (16, ...) targets puts 2 (4, ...) lanes on each
+  // target: lane = slot + part * width, and the parts evaluate interleaved
+  // entries of every list, their sums combined after each flush.
+  int width = kWave;
+  while (width > 1 && (width >> 1) >= g.count) width >>= 1;
+  const int parts = kWave / width;
+  const int slot = lane & (width - 1), part = lane / width;
+  const int64_t i = g.first + slot;
+  const bool valid = slot < g.count && part == 0;        // the lane that owns the result
+  const int64_t ii = slot < g.count ? i : g.first;
   const double px = st.x[ii], py = st.y[ii], pz = st.z[ii];
   const uint64_t ltmask = lanemask_lt();
   const int32_t root = sy.sroot[g.sys], total = sy.send[g.sys];   // this system's tree
@@ -292,7 +310,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   auto flush_cells = [&]() {
     __syncthreads();
     T fx = 0, fy = 0, fz = 0, fp = 0;
-    for (int k = 0; k < nc; ++k) {
+    for (int k = part; k < nc; k += parts) {
       const V4<T> a = ca[k];
       const V4<T> b = cb[k];
       const T qyz = cq[k];
@@ -314,6 +332,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       fz += (mono + c7) * dz - ri5 * qdz;
       if (POT) fp -= a.w * ri + (T)0.5 * dqd * ri5;
     }
+    combine_parts(fx, fy, fz, fp, width);
     Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
     if (POT) P += (double)fp;
     if (COUNT) ncell_tot += nc;
@@ -323,7 +342,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   auto flush_parts = [&]() {
     __syncthreads();
     T fx = 0, fy = 0, fz = 0, fp = 0;
-    for (int k = 0; k < np; ++k) {
+    for (int k = part; k < np; k += parts) {
       const V4<T> b = pl[k];
       const T dx = b.x - tx, dy = b.y - ty, dz = b.z - tz;
       const T r2 = dx * dx + dy * dy + dz * dz + e2;
@@ -334,6 +353,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       fz += mr3 * dz;
       if (POT) fp -= b.w * ri;
     }
+    combine_parts(fx, fy, fz, fp, width);
     Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
     if (POT) P += (double)fp;
     if (COUNT) npart_tot += np;
