@@ -199,7 +199,9 @@ nbx_status nbx_debug_tree_size(nbx_ctx* ctx, int64_t* nodes);
  * (the order of NEXT's recursion), with: depth, leaf flag, body index
  * (upload order, -1 for internal nodes), child-digit mask, geometric
  * centre and half-width (geo4), mass, centre of mass (com3), quadrupole
- * (q6: xx yy zz xy xz yz), and perm = bodies in leaf order. */
+ * (q6: xx yy zz xy xz yz), and perm = bodies in leaf order.  The moments
+ * are NEXT's (octree.h:92-127) in both modes; physics mode lays these
+ * records out on demand from its own build of the same tree. */
 nbx_status nbx_debug_tree(nbx_ctx* ctx, int32_t* depth, int32_t* leaf, int32_t* body,
                           int32_t* cmask, double* geo4, double* mass, double* com3, double* q6,
                           int32_t* perm);

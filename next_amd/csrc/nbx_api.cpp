@@ -284,6 +284,30 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   return NBX_OK;
 }
 
+// Reference node records of the current tree (layout and skip pointers;
+// the moments come from the upward pass): k_emit, the two depth sorts and
+// k_pair.  Needs the scalars of the last build on the host.
+nbx_status ref_layout(nbx_ctx* c) {
+  const int64_t n = c->n, total = c->total_nodes;
+  const int64_t m = (uint32_t)c->hsc->internal_nodes;
+  hipStream_t s = c->stream;
+  // The walk reads node records at 32-bit byte offsets, and one record past
+  // the last node (a successor it loads but never uses).
+  NBXCHK(ensure(c, c->nodes, (size_t)(total + 1) * sizeof(RefNode)));
+  uint32_t* scr = P<uint32_t>(c->scratch);
+  RefNode* nodes = P<RefNode>(c->nodes);
+  launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), sysa(c), c->sc,
+              nodes, P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
+              s);
+  const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
+                                 P<uint32_t>(c->st_val2), m, 8, scr, s);
+  const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
+                                 P<uint32_t>(c->en_val2), m, 8, scr, s);
+  launch_pair(m, P<uint32_t>(sa ? c->st_val2 : c->st_val), P<uint32_t>(ea ? c->en_val2 : c->en_val), nodes,
+              P<int32_t>(c->level_list), s);
+  return NBX_OK;
+}
+
 // Second half: wait for the scalars, then (rarely) the two-word key path,
 // node layout and multipoles.
 nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
@@ -348,28 +372,15 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   const int64_t m = (uint32_t)c->hsc->internal_nodes;
   c->total_nodes = total;
   c->max_depth = c->hsc->max_leaf_depth;
-  // The walk reads node records at 32-bit byte offsets, and one record past
-  // the last node (a successor it loads but never uses).
+  // The reference walk reads 128-byte node records at 32-bit byte offsets.
   if (total >= (int64_t)1 << 25)
     return fail(c, NBX_ERR_ARG, "the tree has " + std::to_string(total) + " nodes; at most 33554431 are supported");
-  NBXCHK(ensure(c, c->nodes, (size_t)(total + 1) * sizeof(RefNode)));
   const size_t mm = (size_t)std::max<int64_t>(m, 1) * 4;
   for (DBuf* b : {&c->st_key, &c->st_val, &c->en_key, &c->en_val, &c->st_key2, &c->st_val2, &c->en_key2,
                   &c->en_val2, &c->level_list})
     NBXCHK(ensure(c, *b, mm));
   NBXCHK(ensure(c, c->scratch,
                 std::max(sort_scratch_words(std::max(c->cap, m)), scan_scratch_words(std::max(c->cap, total))) * 4));
-  scr = P<uint32_t>(c->scratch);
-  RefNode* nodes = P<RefNode>(c->nodes);
-  launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, sy, c->sc, nodes,
-              P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), s);
-  const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
-                                 P<uint32_t>(c->st_val2), m, 8, scr, s);
-  const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
-                                 P<uint32_t>(c->en_val2), m, 8, scr, s);
-  launch_pair(m, P<uint32_t>(sa ? c->st_val2 : c->st_val), P<uint32_t>(ea ? c->en_val2 : c->en_val), nodes,
-              P<int32_t>(c->level_list), s);
-  record(c, EV_BUILD);
   int64_t off[kMaxDepthSlots + 1];
   off[0] = 0;
   for (int d = 0; d < kMaxDepthSlots; ++d) off[d + 1] = off[d] + c->hsc->level_count[d];
@@ -380,16 +391,29 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     NBXCHK(ensure(c, c->gmarks, (size_t)n * 8));
     int32_t* gcnt = P<int32_t>(c->gmarks);
     uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
+    PhysNode* pn = P<PhysNode>(c->pnodes);
     launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
-    launch_phys_leaves(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), St, P<PhysNode>(c->pnodes), gcnt, gmark, s);
+    launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, c->sc, pn,
+                     P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
+                     gcnt, gmark, s);
+    const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
+                                   P<uint32_t>(c->st_val2), m, 8, scr, s);
+    const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
+                                   P<uint32_t>(c->en_val2), m, 8, scr, s);
+    launch_phys_pair(m, P<uint32_t>(sa ? c->st_val2 : c->st_val), P<uint32_t>(ea ? c->en_val2 : c->en_val), pn,
+                     P<int32_t>(c->level_list), s);
+    record(c, EV_BUILD);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
-        launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<PhysNode>(c->pnodes),
-                           c->paux.p, St, n, total, c->cfg.theta, c->cfg.group_size, gcnt, gmark, s);
+        launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], d, pn, c->paux.p,
+                           c->cfg.theta, c->cfg.group_size, gcnt, gmark, s);
     launch_group_compact(n, gcnt, gmark, St.sys, P<Group>(c->tgroups), &c->sc->tree_groups, scr, s);
     NBXCHK(sync_scalars(c));
     c->ntgroups = c->hsc->tree_groups;
   } else {
+    NBXCHK(ref_layout(c));
+    record(c, EV_BUILD);
+    RefNode* nodes = P<RefNode>(c->nodes);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
         launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, c->cfg.theta, s);
@@ -1033,6 +1057,21 @@ nbx_status nbx_debug_tree(nbx_ctx* c, int32_t* depth, int32_t* leaf, int32_t* bo
   if (total == 0) return NBX_OK;
   if (!(depth && leaf && body && cmask && geo4 && mass && com3 && q6 && perm))
     return fail(c, NBX_ERR_ARG, "null output");
+  if (c->physics) {
+    // The physics build writes walk records only: lay out the reference
+    // records of the same tree, with NEXT's moments.
+    NBXCHK(ref_layout(c));
+    int64_t off = 0;
+    for (int d = 0; d < kMaxDepthSlots; ++d) {
+      const int64_t cnt = c->hsc->level_count[d];
+      off += cnt;
+    }
+    for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
+      const int64_t cnt = c->hsc->level_count[d];
+      off -= cnt;
+      if (cnt > 0) launch_upward(P<int32_t>(c->level_list), off, cnt, P<RefNode>(c->nodes), c->cfg.theta, c->stream);
+    }
+  }
   // one staging allocation for all outputs
   const size_t ni = (size_t)total * 4, nd = (size_t)total * 8;
   DBuf tmp;

@@ -54,49 +54,68 @@ __device__ __forceinline__ void mark_group(int32_t f, int32_t c, int32_t* gcnt, 
   gmark[f] = 1u;
 }
 
-// Leaf records (a leaf's FP64 moments are its body's: mass st.m, no
-// quadrupole, zero extent, so no auxiliary record is written).  A leaf that
-// is a system's root is that system's walk group.
-__global__ __launch_bounds__(kBlock) void k_phys_leaves(int64_t n, const int32_t* __restrict__ lcp,
-                                                        const uint32_t* __restrict__ pre_off, StateArrays st,
-                                                        PhysNode* __restrict__ pn, int32_t* __restrict__ gcnt,
-                                                        uint32_t* __restrict__ gmark) {
+// Node layout from the sorted keys, as k_emit does for the reference tree
+// but writing walk records: for body i, the internal nodes that start at it
+// (first body only; the skip pointer comes from k_phys_pair, the moments from
+// k_phys_upward) and its leaf, complete.  A leaf is never accepted as a cell
+// (rc2 = +inf), so its quadrupole slots carry its FP64 mass for the upward
+// pass.  A leaf that is a system's root is that system's walk group.
+__global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* __restrict__ lcp,
+                                                      const uint32_t* __restrict__ pre_off,
+                                                      const uint32_t* __restrict__ end_off, StateArrays st,
+                                                      const DevScalars* __restrict__ sc, PhysNode* __restrict__ pn,
+                                                      uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
+                                                      uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val,
+                                                      int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int li = lcp[i];
   const int lp = i > 0 ? lcp[i - 1] : -1;
-  const uint32_t p = pre_off[i] + (uint32_t)(li > lp ? li - lp : 0);
-  PhysNode nd;
-  nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
-  nd.m = (float)st.m[i];
-  nd.rc2 = INFINITY;
-  nd.qxx = nd.qyy = nd.qxy = nd.qxz = nd.qyz = 0.f;
-  nd.skip = (int32_t)(p + 1);
-  nd.first = (int32_t)i;
-  nd.count = 1;
-  pn[p] = nd;
+  const uint32_t pre = pre_off[i];
+  const int ns = li > lp ? li - lp : 0;
+  const uint32_t sbase = pre - (uint32_t)i;
+  for (int t = 0; t < ns; ++t) {
+    const uint32_t p = pre + t;
+    st_key[sbase + t] = (uint32_t)(lp + 1 + t);
+    st_val[sbase + t] = p;
+    pn[p].first = (int32_t)i;
+  }
+  {
+    const uint32_t p = pre + ns;
+    PhysNode nd;
+    nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
+    const double m = st.m[i];
+    nd.m = (float)m;
+    nd.rc2 = INFINITY;
+    set_leaf_mass(nd, m);
+    nd.qxy = nd.qxz = nd.qyz = 0.f;
+    nd.skip = (int32_t)(p + 1);
+    nd.first = (int32_t)i;
+    nd.count = 1;
+    pn[p] = nd;
+  }
   if (li < 0 && lp < 0) mark_group((int32_t)i, 1, gcnt, gmark);
+  const int ne = lp > li ? lp - li : 0;
+  if (ne) {
+    const uint32_t skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
+    const uint32_t eo = end_off[i];
+    for (int t = 0; t < ne; ++t) {
+      en_key[eo + t] = (uint32_t)(li + 1 + t);
+      en_val[eo + t] = skipv;
+    }
+  }
 }
 
-// FP64 moments of child k: from its auxiliary record, or, for a leaf, from
-// its body.
-struct ChildMoments {
-  double m, bmax, q[6], lo[3], hi[3];
-};
-__device__ __forceinline__ void child_moments(int32_t k, const PhysNode* pn, const PhysAux* aux, const double* mass,
-                                              ChildMoments& o) {
-  if (pn[k].count == 1) {
-    o.m = mass[pn[k].first];
-    o.bmax = 0.0;
-    for (int j = 0; j < 6; ++j) o.q[j] = 0.0;
-    o.lo[0] = o.hi[0] = pn[k].cx; o.lo[1] = o.hi[1] = pn[k].cy; o.lo[2] = o.hi[2] = pn[k].cz;
-  } else {
-    const PhysAux& a = aux[k];
-    o.m = a.m;
-    o.bmax = a.bmax;
-    for (int j = 0; j < 6; ++j) o.q[j] = a.q[j];
-    for (int d = 0; d < 3; ++d) { o.lo[d] = a.lo[d]; o.hi[d] = a.hi[d]; }
-  }
+// The k-th internal node of depth d in start order is the k-th in end order:
+// its skip pointer, and the per-depth lists of internal nodes.
+__global__ __launch_bounds__(kBlock) void k_phys_pair(int64_t m, const uint32_t* __restrict__ st_val,
+                                                      const uint32_t* __restrict__ en_val,
+                                                      PhysNode* __restrict__ pn, int32_t* __restrict__ level_list) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m) return;
+  const uint32_t p = st_val[k];
+  pn[p].skip = (int32_t)en_val[k];
+  level_list[k] = (int32_t)p;
 }
 
 // Internal nodes of one depth from their children (FP64), and the walk
@@ -106,37 +125,37 @@ __device__ __forceinline__ void child_moments(int32_t k, const PhysNode* pn, con
 // Every body lands in exactly one group and a group's bodies lie inside one
 // node, so groups are compact in space where runs of G consecutive Morton
 // keys are not (such a run can straddle two distant cells).
-__global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg,
-                                                        int64_t cnt, const RefNode* __restrict__ rn,
-                                                        PhysNode* __restrict__ pn, PhysAux* __restrict__ aux,
-                                                        const double* __restrict__ mass,
-                                                        int64_t n, int64_t total, double inv_theta, int G,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg,
+                                                        int64_t cnt, bool root_level, PhysNode* __restrict__ pn,
+                                                        PhysAux* __restrict__ aux, double inv_theta, int G,
                                                         int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= cnt) return;
   const int32_t id = level_list[beg + k];
-  const int32_t end = rn[id].skip;
-  const int32_t first = rn[id].first;
+  const int32_t end = pn[id].skip;
+  const int32_t first = pn[id].first;
   // One walk of the child chain (at most 8 children) records the child
-  // indices; -1 marks an empty slot.
-  int32_t chv[8];
+  // indices (-1: no child) and sizes and sums mass and mass moment; the
+  // second pass reads the same (now cached) records again, which keeps the
+  // kernel's registers, and so its occupancy, where a latency-bound pass
+  // needs them.
+  int32_t chv[8], cnv[8];
+  double M = 0, sx = 0, sy = 0, sz = 0;
+  int32_t count = 0;
   int32_t c = id + 1;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    chv[j] = -1;
+    chv[j] = -1; cnv[j] = 0;
     if (c < end) {
+      const PhysNode& ch = pn[c];
       chv[j] = c;
-      c = pn[c].skip;
+      cnv[j] = ch.count;
+      const double cm = ch.count == 1 ? leaf_mass(ch) : aux[c].m;
+      M += cm;
+      sx += ch.cx * cm; sy += ch.cy * cm; sz += ch.cz * cm;
+      count += ch.count;
+      c = ch.skip;
     }
-  }
-  double M = 0, sx = 0, sy = 0, sz = 0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    if (chv[j] < 0) continue;
-    const int32_t kc = chv[j];
-    const double cm = pn[kc].count == 1 ? mass[pn[kc].first] : aux[kc].m;
-    M += cm;
-    sx += pn[kc].cx * cm; sy += pn[kc].cy * cm; sz += pn[kc].cz * cm;
   }
   double cx, cy, cz;
   if (M > 0) { cx = sx / M; cy = sy / M; cz = sz / M; }
@@ -147,62 +166,73 @@ __global__ __launch_bounds__(kBlock) void k_phys_upward(const int32_t* __restric
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     if (chv[j] < 0) continue;
-    const int32_t kc = chv[j];
-    ChildMoments a;
-    child_moments(kc, pn, aux, mass, a);
-    const double rx = pn[kc].cx - cx, ry = pn[kc].cy - cy, rz = pn[kc].cz - cz;
+    const PhysNode& ch = pn[chv[j]];
+    const double px = ch.cx, py = ch.cy, pz = ch.cz;
+    const double rx = px - cx, ry = py - cy, rz = pz - cz;
     const double r2 = rx * rx + ry * ry + rz * rz;
-    const double mc = a.m;
-    q[0] += a.q[0] + mc * (3 * rx * rx - r2);
-    q[1] += a.q[1] + mc * (3 * ry * ry - r2);
-    q[2] += a.q[2] + mc * (3 * rz * rz - r2);
-    q[3] += a.q[3] + mc * 3 * rx * ry;
-    q[4] += a.q[4] + mc * 3 * rx * rz;
-    q[5] += a.q[5] + mc * 3 * ry * rz;
-    const double b = sqrt(r2) + a.bmax;
-    if (b > bsum) bsum = b;
-    for (int d = 0; d < 3; ++d) {
-      if (a.lo[d] < lo[d]) lo[d] = a.lo[d];
-      if (a.hi[d] > hi[d]) hi[d] = a.hi[d];
+    double b = sqrt(r2);
+    double mc;
+    if (cnv[j] == 1) {                                     // a body: a point
+      mc = leaf_mass(ch);
+      lo[0] = fmin(lo[0], px); hi[0] = fmax(hi[0], px);
+      lo[1] = fmin(lo[1], py); hi[1] = fmax(hi[1], py);
+      lo[2] = fmin(lo[2], pz); hi[2] = fmax(hi[2], pz);
+    } else {                                               // a cell: its moments and box
+      const PhysAux& a = aux[chv[j]];
+      mc = a.m;
+      for (int t = 0; t < 6; ++t) q[t] += a.q[t];
+      b += a.bmax;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = fmin(lo[d], a.lo[d]);
+        hi[d] = fmax(hi[d], a.hi[d]);
+      }
     }
+    q[0] += mc * (3 * rx * rx - r2);
+    q[1] += mc * (3 * ry * ry - r2);
+    q[2] += mc * (3 * rz * rz - r2);
+    q[3] += mc * 3 * rx * ry;
+    q[4] += mc * 3 * rx * rz;
+    q[5] += mc * 3 * ry * rz;
+    if (b > bsum) bsum = b;
   }
   // bmax: the farthest corner of the bodies' box from the centre of mass,
   // or the children's bound if that is smaller.
   const double ex = fmax(cx - lo[0], hi[0] - cx), ey = fmax(cy - lo[1], hi[1] - cy), ez = fmax(cz - lo[2], hi[2] - cz);
   double bmax = sqrt(ex * ex + ey * ey + ez * ez);
   if (bsum < bmax) bmax = bsum;
-  const int32_t next_first = end < total ? rn[end].first : (int32_t)n;
   PhysNode nd;
   nd.cx = cx; nd.cy = cy; nd.cz = cz;
   nd.m = (float)M;
-  nd.count = next_first - first;
+  nd.count = count;
   // Cells of very few bodies cost more as a multipole than as bodies.
   const double rc = bmax * inv_theta;
-  nd.rc2 = nd.count < 4 ? INFINITY : (float)(rc * rc * (1.0 + 1e-6));
+  nd.rc2 = count < 4 ? INFINITY : (float)(rc * rc * (1.0 + 1e-6));
   nd.qxx = (float)q[0]; nd.qyy = (float)q[1];
   nd.qxy = (float)q[3]; nd.qxz = (float)q[4]; nd.qyz = (float)q[5];
   nd.skip = end;
   nd.first = first;
   pn[id] = nd;
   // walk groups
-  if (nd.count > G) {
-    int32_t run_first = 0, run = 0;
+  if (count > G) {
+    int32_t run_first = 0, run = 0, f = first;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (chv[j] < 0) continue;
-      const int32_t cc = pn[chv[j]].count;
+      const int32_t cc = cnv[j];
+      const int32_t cf = f;                                // children's bodies follow one another
+      f += cc;
+      if (cc == 0) continue;
       if (cc > G) {                                        // packs its own children
         if (run > 0) mark_group(run_first, run, gcnt, gmark);
         run = 0;
         continue;
       }
       if (run > 0 && run + cc > G) { mark_group(run_first, run, gcnt, gmark); run = 0; }
-      if (run == 0) run_first = pn[chv[j]].first;
+      if (run == 0) run_first = cf;
       run += cc;
     }
     if (run > 0) mark_group(run_first, run, gcnt, gmark);
-  } else if (rn[id].depth == 0) {
-    mark_group(first, nd.count, gcnt, gmark);
+  } else if (root_level) {
+    mark_group(first, count, gcnt, gmark);
   }
   PhysAux a;
   a.m = M;
@@ -591,19 +621,25 @@ __global__ __launch_bounds__(kBlock) void k_group_compact(int64_t n, const int32
   if (c != 0) groups[gidx[i]] = Group{(int32_t)i, c, body_sys[i], 0};
 }
 
-void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st, PhysNode* pn,
-                        int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
+void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
+                      const StateArrays& st, const DevScalars* sc, PhysNode* pn, uint32_t* st_key, uint32_t* st_val,
+                      uint32_t* en_key, uint32_t* en_val, int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_phys_leaves, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, st, pn, gcnt,
-                       gmark);
+    hipLaunchKernelGGL(k_phys_emit, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sc,
+                       pn, st_key, st_val, en_key, en_val, gcnt, gmark);
 }
 
-void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
-                        void* aux, const StateArrays& st, int64_t n, int64_t total, double theta, int group_size,
-                        int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
+void launch_phys_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, PhysNode* pn, int32_t* level_list,
+                      hipStream_t s) {
+  if (m > 0)
+    hipLaunchKernelGGL(k_phys_pair, dim3(grid_of(m, kBlock)), dim3(kBlock), 0, s, m, st_val, en_val, pn, level_list);
+}
+
+void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, int depth, PhysNode* pn, void* aux,
+                        double theta, int group_size, int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
   if (cnt > 0)
-    hipLaunchKernelGGL(k_phys_upward, dim3(grid_of(cnt, kBlock)), dim3(kBlock), 0, s, level_list, beg, cnt, rn, pn,
-                       reinterpret_cast<PhysAux*>(aux), st.m, n, total, 1.0 / theta, group_size, gcnt, gmark);
+    hipLaunchKernelGGL(k_phys_upward, dim3(grid_of(cnt, kBlock)), dim3(kBlock), 0, s, level_list, beg, cnt,
+                       depth == 0, pn, reinterpret_cast<PhysAux*>(aux), 1.0 / theta, group_size, gcnt, gmark);
 }
 
 void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const int32_t* body_sys, Group* groups,

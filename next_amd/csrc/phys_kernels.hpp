@@ -25,6 +25,15 @@ struct alignas(64) PhysNode {
 };
 static_assert(sizeof(PhysNode) == 64, "PhysNode layout");
 
+// A leaf is never accepted as a cell (rc2 = +inf), so its first two
+// quadrupole slots hold its FP64 mass, which the upward pass reads.
+__host__ __device__ inline void set_leaf_mass(PhysNode& nd, double m) { __builtin_memcpy(&nd.qxx, &m, 8); }
+__host__ __device__ inline double leaf_mass(const PhysNode& nd) {
+  double m;
+  __builtin_memcpy(&m, &nd.qxx, 8);
+  return m;
+}
+
 constexpr int kPhysBlock = 64;      // one wave = one group of targets
 constexpr int kPhysCells = 128;     // per-wave LDS cell list
 constexpr int kPhysParts = 256;     // per-wave LDS body list
@@ -34,17 +43,21 @@ constexpr int kPhysAuxBytes = 128;  // FP64 moments per node (mass, bmax, Q, box
 
 // Walk groups are marked per first body in gcnt (body count) and gmark
 // (1), both zeroed beforehand, then compacted in body order.
-// Leaves: one PhysNode per body; a leaf that is a system root is a one-body
-// group.
-void launch_phys_leaves(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const StateArrays& st, PhysNode* pn,
-                        int32_t* gcnt, uint32_t* gmark, hipStream_t s);
+// Node layout from the sorted keys (the arguments of launch_emit): leaf
+// records complete, internal nodes' first body, the depth-sort keys; a leaf
+// that is a system root is a one-body group.
+void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
+                      const StateArrays& st, const DevScalars* sc, PhysNode* pn, uint32_t* st_key, uint32_t* st_val,
+                      uint32_t* en_key, uint32_t* en_val, int32_t* gcnt, uint32_t* gmark, hipStream_t s);
+// Skip pointers and per-depth internal node lists from the two depth sorts.
+void launch_phys_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, PhysNode* pn, int32_t* level_list,
+                      hipStream_t s);
 // Internal nodes of one depth from their children (FP64 accumulation), and
 // the walk groups of at most group_size bodies each node packs (see
 // k_phys_upward).  `aux`: kPhysAuxBytes per node of FP64 moments (mass,
 // bmax, quadrupole, bounding box of the bodies), internal nodes only.
-void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, const RefNode* rn, PhysNode* pn,
-                        void* aux, const StateArrays& st, int64_t n, int64_t total, double theta, int group_size,
-                        int32_t* gcnt, uint32_t* gmark, hipStream_t s);
+void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, int depth, PhysNode* pn, void* aux,
+                        double theta, int group_size, int32_t* gcnt, uint32_t* gmark, hipStream_t s);
 // The group table from the marks (gmark is overwritten by its scan);
 // *ngroups (device) receives the count; scratch as for scan_u32 over n.
 void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const int32_t* body_sys, Group* groups,
