@@ -11,10 +11,11 @@ resident in HBM.
 Prints one JSON line: ms per KDK step, the walk kernel's roofline (FP64
 VALU; algorithmic flops 19 per particle-particle and 55 per particle-cell
 interaction, counted on the device; HBM traffic from the committed PMC
-summary), the per-phase split with the HBM rate of the streaming phases, a
-bounded sample of the batched-ensemble and serving workloads (physics mode),
-and NEXT's own CPU code timed on this host on a bounded sample of the
-headline workload.
+summary), the per-phase split with the HBM rate of the streaming phases, the
+mixed-precision physics mode on the same workload (step time, force error
+against the FP64 direct sum, energy drift), a bounded sample of the
+batched-ensemble and serving workloads (physics mode), and NEXT's own CPU
+code timed on this host on a bounded sample of the headline workload.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-extra] [--no-cpu]
 """
@@ -104,6 +105,48 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
                           "trial_ms_per_step": {f"{lf}/{g}": round(float(m), 3) for (lf, g), m in zip(cands, trial_ms)}},
             "workload": f"{nsys} Plummer spheres (DF samples), N log-uniform [1024, 65536], theta 0.5 (bmax criterion), "
                         "eps 0.01, dt 1e-3, FP32 walk / FP64 state"}
+
+
+def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) -> dict:
+    """Mixed-precision physics mode (FP32 walk, FP64 state) on the headline
+    workload: cold uniform sphere of n bodies, eps 0.005, fixed dt 5e-4.  Per
+    theta: ms per KDK step, force error against the FP64 direct sum on 20,000
+    sampled bodies, and per-body throughput relative to the reference-mode
+    step (ref_ms); |dE/E| (FP64 direct-sum energies) over energy_steps steps
+    at the first theta."""
+    from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Engine
+
+    p = ic.uniform_sphere(n, seed=2)
+    with Engine(mode=MODE_PHYSICS, theta=0.5, eps=0.005, precision=PREC_FP64) as e:
+        e.upload(p)
+        a_ref, _ = e.direct()
+    sub = np.random.Generator(np.random.PCG64(0)).choice(n, min(n, 20000), replace=False)
+    rows, energy = [], None
+    for j, theta in enumerate(thetas):
+        with Engine(mode=MODE_PHYSICS, theta=theta, eps=0.005, precision=PREC_MIXED, leaf_size=16, group_size=64,
+                    adaptive_dt=False) as e:
+            e.upload(p)
+            a = e.accel()
+            d = np.linalg.norm(a_ref[sub], axis=1)
+            err = np.linalg.norm(a[sub] - a_ref[sub], axis=1) / np.where(d > 0, d, 1.0)
+            e.run(5e-4, 2)
+            t0 = time.perf_counter()
+            e.run(5e-4, 10)
+            ms = (time.perf_counter() - t0) * 1e2
+            rows.append({"theta": theta, "ms_per_step": round(ms, 4), "median_rel_force_err": float(np.median(err)),
+                         "p99_rel_force_err": float(np.percentile(err, 99)), "body_steps_per_s": n / (ms * 1e-3),
+                         "vs_reference_path": round(ref_ms / ms, 3)})
+            if j == 0 and energy_steps > 0:
+                e.upload(p)
+                e0 = e.energy(direct=True)
+                e.run(5e-4, energy_steps)
+                e1 = e.energy(direct=True)
+                E0, E1 = e0[0] + e0[1], e1[0] + e1[1]
+                energy = {"theta": theta, "steps": energy_steps, "E0": E0, "E1": E1,
+                          "abs_dE_over_E": abs(E1 - E0) / abs(E0)}
+        log("physics theta %g: %.3f ms/step, median force error %.2e" % (theta, ms, rows[-1]["median_rel_force_err"]))
+    return {"workload": f"cold uniform sphere N={n}, eps 0.005, fixed dt 5e-4, leaf 16, group 64, FP32 walk / FP64 state",
+            "by_theta": rows, "energy": energy}
 
 
 def serving(nreq: int, rate: float, seed: int = 5) -> dict:
@@ -266,6 +309,7 @@ def main() -> int:
     }
     log("headline: %.3f ms/step" % ms_per_step)
     if not args.no_extra:
+        out["physics"] = physics(args.n, ms_per_step)
         out["ensemble"] = ensemble(args.ensemble_systems, steps=5, warmup=1)
         log("ensemble: %.1f ms/step" % out["ensemble"]["ms_per_step"])
         out["serving"] = []
