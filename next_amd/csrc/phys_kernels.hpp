@@ -35,10 +35,12 @@ __host__ __device__ inline double leaf_mass(const PhysNode& nd) {
 }
 
 constexpr int kPhysBlock = 64;      // one wave = one group of targets
+// Per-wave LDS (7.4 KB with these sizes) bounds the walk's occupancy at
+// about 5 waves per SIMD; a 256-entry stack and body list (11 KB) allowed 3.
 constexpr int kPhysCells = 128;     // per-wave LDS cell list
-constexpr int kPhysParts = 256;     // per-wave LDS body list
+constexpr int kPhysParts = 128;     // per-wave LDS body list
 constexpr int kPhysMaxLeaf = 64;    // bucket size limit (one body per lane)
-constexpr int kPhysStack = 256;     // per-wave LDS stack of donated node ranges
+constexpr int kPhysStack = 32;      // per-wave LDS stack of donated node ranges
 constexpr int kPhysAuxBytes = 128;  // FP64 moments per node (mass, bmax, Q, box)
 
 // Walk groups are marked per first body in gcnt (body count) and gmark

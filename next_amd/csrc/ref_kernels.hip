@@ -519,8 +519,10 @@ __device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes,
 // of each lane's oldest entries one evaluation round takes.  Evaluating only
 // the oldest entries when some lane's ring is full keeps more lanes busy per
 // evaluation step than draining every ring: lanes fill at different rates.
-constexpr int kWalkList = 32;
-constexpr int kWalkFlush = 8;
+// 16 entries (4 KB of LDS per wave) leave the occupancy to the registers:
+// 6 waves per SIMD, against 5 with 32 entries, measured 4 % faster.
+constexpr int kWalkList = 16;
+constexpr int kWalkFlush = 4;
 static_assert((kWalkList & (kWalkList - 1)) == 0 && kWalkFlush <= kWalkList, "walk ring");
 
 // Two phases per wave.  Traversal: every lane follows its own depth-first
@@ -619,43 +621,50 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
   int32_t next = valid ? root : INT_MAX;
   int32_t cur = root;
   NodeHead h = load_head(nodes, root);
-  while (cur < total) {
-    // successors, loaded while this node is processed
-    // (the node array has one spare record after the last node, so both
-    // successors, at most total, are readable)
-    const int32_t sk = h.skip > cur ? h.skip : cur + 1;   // skip > cur by construction
-    const NodeHead h1 = load_head(nodes, (uint32_t)cur + 1u);
-    const NodeHead h2 = load_head(nodes, (uint32_t)sk);
-    bool open = false;
-    if (next == cur) {
-      if (h.m == 0.0 || h.body == me) {                    // octree.h:134-135
-        next = sk;
-      } else {
-        const bool leaf = h.body >= 0;
-        bool accept = leaf;
-        if (!leaf) {
-          const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
-          const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
-          if (r2e > h.acc2) accept = true;
-          else if (r2e < h.open2) accept = false;
-          else accept = (nodes[cur].size / sqrt(r2e)) < theta;
-        }
-        if (accept) {
-          mine[((head + len) & (kWalkList - 1)) * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
-          ++len;
+  // Traversal runs until some lane's ring is full (or the tree is done),
+  // then one evaluation round; the running sums are not touched inside the
+  // traversal loop, which keeps them out of its loop-carried registers.
+  for (;;) {
+    while (cur < total) {
+      // successors, loaded while this node is processed
+      // (the node array has one spare record after the last node, so both
+      // successors, at most total, are readable)
+      const int32_t sk = h.skip > cur ? h.skip : cur + 1;   // skip > cur by construction
+      const NodeHead h1 = load_head(nodes, (uint32_t)cur + 1u);
+      const NodeHead h2 = load_head(nodes, (uint32_t)sk);
+      bool open = false;
+      if (next == cur) {
+        if (h.m == 0.0 || h.body == me) {                    // octree.h:134-135
           next = sk;
-          if (COUNT) { if (leaf) ++npp; else ++npc; }
         } else {
-          open = true;                                      // first child
-          next = cur + 1;
-          if (COUNT) ++nop;
+          const bool leaf = h.body >= 0;
+          bool accept = leaf;
+          if (!leaf) {
+            const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
+            const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
+            if (r2e > h.acc2) accept = true;
+            else if (r2e < h.open2) accept = false;
+            else accept = (nodes[cur].size / sqrt(r2e)) < theta;
+          }
+          if (accept) {
+            mine[((head + len) & (kWalkList - 1)) * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
+            ++len;
+            next = sk;
+            if (COUNT) { if (leaf) ++npp; else ++npc; }
+          } else {
+            open = true;                                      // first child
+            next = cur + 1;
+            if (COUNT) ++nop;
+          }
         }
       }
+      if (COUNT) ++nit;
+      if (__ballot(open)) { cur = cur + 1; h = h1; }
+      else { cur = sk; h = h2; }
+      if (__ballot(len == kWalkList)) break;
     }
-    if (COUNT) ++nit;
-    if (__ballot(open)) { cur = cur + 1; h = h1; }
-    else { cur = sk; h = h2; }
-    if (__ballot(len == kWalkList)) evaluate(kWalkFlush);
+    if (cur >= total) break;
+    evaluate(kWalkFlush);
   }
   evaluate(kWalkList);
   (void)h;
