@@ -604,12 +604,17 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
           const double inv7 = inv5 * (dinv * dinv);
           const double q = qxx * dx * dx + qyy * dy * dy + qzz * dz * dz +
                            2.0 * (qxy * dx * dy + qxz * dx * dz + qyz * dy * dz);
-          const double qrx = 2.0 * (qxx * dx + qxy * dy + qxz * dz);
-          const double qry = 2.0 * (qxy * dx + qyy * dy + qyz * dz);
-          const double qrz = 2.0 * (qxz * dx + qyz * dy + qzz * dz);
-          a0 = a0 + 0.5 * (qrx * inv5 - 5.0 * q * inv7 * dx);
-          a1 = a1 + 0.5 * (qry * inv5 - 5.0 * q * inv7 * dy);
-          a2 = a2 + 0.5 * (qrz * inv5 - 5.0 * q * inv7 * dz);
+          // 0.5 * (Qr * inv5 - 5 q inv7 d) with Qr = 2 (Q d) (octree.h:166-173),
+          // the powers of two moved inside: doubling and halving are exact,
+          // so (Q d) inv5 - ((2.5 q) inv7) d rounds to the same bits for every
+          // intermediate in the normal range (magnitudes above 2.2e-308).
+          const double w = 2.5 * q * inv7;
+          const double sx = qxx * dx + qxy * dy + qxz * dz;
+          const double sy = qxy * dx + qyy * dy + qyz * dz;
+          const double sz = qxz * dx + qyz * dy + qzz * dz;
+          a0 = a0 + (sx * inv5 - w * dx);
+          a1 = a1 + (sy * inv5 - w * dy);
+          a2 = a2 + (sz * inv5 - w * dz);
         }
       }
     }
