@@ -99,12 +99,65 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
         e.run(1e-3, steps)
         dt = time.perf_counter() - t0
     total = int(sum(sizes))
+    target = ensemble_target(systems, best)
     return {"systems": nsys, "bodies": total, "steps": steps, "ensemble_steps_per_s": steps / dt,
             "ms_per_step": dt * 1e3 / steps, "body_steps_per_s": total * steps / dt,
             "autotuned": {"leaf_size": best[0], "group_size": best[1],
                           "trial_ms_per_step": {f"{lf}/{g}": round(float(m), 3) for (lf, g), m in zip(cands, trial_ms)}},
             "workload": f"{nsys} Plummer spheres (DF samples), N log-uniform [1024, 65536], theta 0.5 (bmax criterion), "
-                        "eps 0.01, dt 1e-3, FP32 walk / FP64 state"}
+                        "eps 0.01, dt 1e-3, FP32 walk / FP64 state",
+            "at_target_accuracy": target}
+
+
+def ensemble_target(systems, best, theta: float = 0.35, energy_steps: int = 200) -> dict:
+    """The same ensemble at an opening angle whose median force error against
+    the FP64 direct sum is at most 1e-4: force error on 20,000 sampled bodies,
+    ms per step, per-system |dE/E| (FP64 direct-sum energies) over
+    energy_steps steps, and the per-body throughput against NEXT's plain FP64
+    algorithm (reference mode) run on the same systems one at a time."""
+    from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Engine
+
+    with Engine(mode=MODE_PHYSICS, precision=PREC_FP64, eps=0.01, theta=0.5) as e:
+        e.upload_batch(systems)
+        a_ref, _ = e.direct()
+    total = len(a_ref)
+    sub = np.random.Generator(np.random.PCG64(0)).choice(total, min(total, 20000), replace=False)
+    with Engine(mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, theta=theta, leaf_size=best[0],
+                group_size=best[1], adaptive_dt=False) as e:
+        e.upload_batch(systems)
+        a = e.accel()
+        d = np.linalg.norm(a_ref[sub], axis=1)
+        err = np.linalg.norm(a[sub] - a_ref[sub], axis=1) / np.where(d > 0, d, 1.0)
+        e.run(1e-3, 2)
+        t0 = time.perf_counter()
+        e.run(1e-3, 5)
+        ms = (time.perf_counter() - t0) / 5 * 1e3
+        e.upload_batch(systems)
+        en0 = e.energy_systems(direct=True)
+        e.run(1e-3, energy_steps)
+        en1 = e.energy_systems(direct=True)
+    E0, E1 = en0[:, 0] + en0[:, 1], en1[:, 0] + en1[:, 1]
+    drift = np.abs(E1 - E0) / np.abs(E0)
+    log("ensemble at theta %g: %.2f ms/step, median force error %.2e, max |dE/E| %.2e"
+        % (theta, ms, float(np.median(err)), float(drift.max())))
+    seq = 0.0
+    with Engine(theta=0.5) as e:       # reference mode: NEXT's FP64 algorithm, adaptive dt
+        for p in systems:
+            e.upload(p)
+            e.run(1e-3, 1)
+            t0 = time.perf_counter()
+            e.run(1e-3, 3)
+            seq += (time.perf_counter() - t0) / 3
+    log("plain FP64 path, systems one at a time: %.1f ms per ensemble step" % (seq * 1e3))
+    return {"theta": theta, "median_rel_force_err": float(np.median(err)),
+            "p99_rel_force_err": float(np.percentile(err, 99)), "ms_per_step": round(ms, 3),
+            "body_steps_per_s": total / (ms * 1e-3),
+            "energy": {"steps": energy_steps, "median_abs_dE_over_E": float(np.median(drift)),
+                       "max_abs_dE_over_E": float(drift.max())},
+            "plain_fp64_one_system_at_a_time": {"ms_per_ensemble_step": round(seq * 1e3, 3),
+                                                "body_steps_per_s": total / seq,
+                                                "path": "reference mode (NEXT's FP64 algorithm), theta 0.5"},
+            "vs_plain_fp64_path": round(seq * 1e3 / ms, 2)}
 
 
 def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) -> dict:
