@@ -257,8 +257,10 @@ def phase_rooflines(n: int, tim: dict, k: int) -> dict:
     return out
 
 
-def load_traffic() -> dict | None:
-    path = os.path.join(ROOT, "profiles", "walk_traffic.json")
+def load_profile(name: str) -> dict | None:
+    """A committed rocprofv3 summary of the walk kernel (HBM traffic or VALU
+    issue counters), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -323,7 +325,8 @@ def main() -> int:
     walk_ms = tim["walk_ms"] / k
     flops = FLOPS_PP * cnt["n_pp"] + FLOPS_PC * cnt["n_pc"]
     achieved = flops / (walk_ms * 1e-3) / 1e12 if walk_ms > 0 else 0.0
-    traffic = load_traffic()
+    traffic = load_profile("walk_traffic.json")
+    valu = load_profile("walk_valu.json")
 
     out = {
         "metric": "ms per KDK step at N=1M, \u03b8=0.5; ensemble steps/s + p50/p99 latency",
@@ -347,7 +350,10 @@ def main() -> int:
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "walk_ms": round(walk_ms, 4), "flops_per_launch": flops,
-                     "n_pp": cnt["n_pp"], "n_pc": cnt["n_pc"]},
+                     "n_pp": cnt["n_pp"], "n_pc": cnt["n_pc"],
+                     "valu_issue": ({"busy_frac": valu["derived"]["valu_busy_frac"],
+                                     "instructions_per_slot": valu["derived"]["valu_instructions_per_slot"],
+                                     "source": "profiles/walk_valu.json"} if valu else None)},
         "interactions_per_s": (cnt["n_pp"] + cnt["n_pc"]) / (walk_ms * 1e-3) if walk_ms > 0 else 0.0,
         "walk_counters": {"n_open": cnt["n_open"], "traversal_wave_steps": cnt["n_iter"],
                           "traversal_lane_efficiency": round((cnt["n_pp"] + cnt["n_pc"] + cnt["n_open"])
