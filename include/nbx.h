@@ -104,6 +104,16 @@ void nbx_destroy(nbx_ctx* ctx);
 const char* nbx_last_error(const nbx_ctx* ctx);
 nbx_status nbx_set_theta(nbx_ctx* ctx, double theta);
 
+/* Physics mode: opening criterion relative to each group's previous
+ * acceleration (no counterpart in NEXT, whose criterion is geometric).  A
+ * cell that passes the geometric test (distance to the group's box beyond
+ * bmax / theta) is accepted only if m bmax^3 <= alpha |a|min d^5, where
+ * |a|min is the smallest |a| of the group at the start of the step and d the
+ * distance.  Applies to the walks inside nbx_step / nbx_run after the first
+ * evaluation (which has no previous acceleration); alpha = 0 (the default)
+ * turns it off.  NBX_ERR_ARG for alpha < 0, non-finite, or reference mode. */
+nbx_status nbx_set_alpha(nbx_ctx* ctx, double alpha);
+
 /* Copy n particles in (Particle SoA of particle.h:22-52).  n may be 0. */
 nbx_status nbx_upload(nbx_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                       const double* vx, const double* vy, const double* vz, const double* m,

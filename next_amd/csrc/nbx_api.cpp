@@ -73,6 +73,10 @@ struct nbx_ctx {
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
   bool physics = false;       // NBX_MODE_PHYSICS
+  double alpha = 0.0;         // physics mode: relative opening criterion (0: geometric only)
+  DBuf amag[2];               // |a| per body from the opening kick, in key order (FP32)
+  int acur = 0;
+  bool amag_live = false;     // amag[acur] follows the bodies through this step's sort
   bool acc_valid = false;     // acc holds a(x) of the current positions
   bool tree_valid = false;    // nodes hold the tree of the current positions
   const uint64_t* skey = nullptr;
@@ -267,6 +271,10 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   launch_gather_state(n, perm, St, view(c, 1 - c->cur), s);
   c->cur = 1 - c->cur;
   St = view(c, c->cur);
+  if (c->amag_live) {
+    launch_gather_u32(n, perm, P<uint32_t>(c->amag[c->acur]), P<uint32_t>(c->amag[1 - c->acur]), s);
+    c->acur = 1 - c->acur;
+  }
   if (S > 1) {   // keys of the system-major order
     launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
     skey = P<uint64_t>(c->key[0]);
@@ -350,6 +358,10 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     launch_gather_state(n, pf, St, view(c, 1 - c->cur), s);
     c->cur = 1 - c->cur;
     St = view(c, c->cur);
+    if (c->amag_live) {
+      launch_gather_u32(n, pf, P<uint32_t>(c->amag[c->acur]), P<uint32_t>(c->amag[1 - c->acur]), s);
+      c->acur = 1 - c->acur;
+    }
     launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
     launch_keys2(n, St, sy, P<uint64_t>(c->key2[0]), s);
     skey = P<uint64_t>(c->key[0]);
@@ -439,7 +451,8 @@ nbx_status walk(nbx_ctx* c, bool kick) {
   if (c->physics)
     launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                      c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr,
-                     c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64, c->stream);
+                     c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64,
+                     c->amag_live ? P<float>(c->amag[c->acur]) : nullptr, c->alpha, c->cfg.theta, c->stream);
   else
     launch_walk(P<Group>(c->groups), c->ngroups, P<RefNode>(c->nodes), view(c, c->cur), sysa(c), c->cfg.theta,
                 P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
@@ -493,15 +506,23 @@ nbx_status ensure_accel(nbx_ctx* c) {
 // One KDK step of every system with the dt held on the device.
 nbx_status step_device(nbx_ctx* c) {
   if (c->n == 0) return NBX_OK;
+  c->amag_live = false;                 // the first evaluation uses the geometric criterion
   NBXCHK(ensure_accel(c));
+  const bool rel = c->physics && c->alpha > 0.0;
+  if (rel)
+    for (DBuf& b : c->amag) NBXCHK(ensure(c, b, (size_t)c->n * sizeof(float)));
   record(c, EV_START);
   launch_kick_drift(c->n, view(c, c->cur), P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), sysa(c),
-                    c->stream);
+                    rel ? P<float>(c->amag[c->acur]) : nullptr, c->stream);
   c->acc_valid = false;
   c->tree_valid = false;
+  c->amag_live = rel;
   record(c, EV_KICK);
-  NBXCHK(build_tree(c));
-  NBXCHK(walk(c, true));
+  const nbx_status st = build_tree(c);
+  if (st != NBX_OK) { c->amag_live = false; return st; }
+  const nbx_status sw = walk(c, true);
+  c->amag_live = false;
+  NBXCHK(sw);
   return finish_timing(c, true);
 }
 
@@ -632,6 +653,7 @@ void nbx_destroy(nbx_ctx* c) {
   for (auto& b : c->key) release(b);
   for (auto& b : c->idx) release(b);
   for (auto& b : c->key2) release(b);
+  for (auto& b : c->amag) release(b);
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
                   &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout, &c->s_root, &c->s_dt, &c->s_v2,
@@ -703,6 +725,14 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
   c->cfg = keep;
   c->cfg.leaf_size = leaf[best];
   c->cfg.group_size = group[best];
+  return NBX_OK;
+}
+
+nbx_status nbx_set_alpha(nbx_ctx* c, double alpha) {
+  if (!c) return NBX_ERR_ARG;
+  if (!(alpha >= 0.0 && alpha < INFINITY)) return fail(c, NBX_ERR_ARG, "alpha must be finite and >= 0");
+  if (alpha > 0.0 && !c->physics) return fail(c, NBX_ERR_ARG, "the relative opening criterion is a physics-mode option");
+  c->alpha = alpha;
   return NBX_OK;
 }
 
@@ -904,7 +934,8 @@ static nbx_status tree_potential(nbx_ctx* c) {
   }
   launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                    c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]),
-                   P<double>(c->pot), c->sc, false, false, c->cfg.precision == NBX_PREC_FP64, c->stream);
+                   P<double>(c->pot), c->sc, false, false, c->cfg.precision == NBX_PREC_FP64, nullptr, 0.0,
+                   c->cfg.theta, c->stream);
   HIPCHK(c, hipGetLastError());
   c->acc_valid = true;
   return NBX_OK;

@@ -287,7 +287,8 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
                                                           SysArrays sy, double eps, int nleaf,
                                                           double* __restrict__ ax, double* __restrict__ ay,
                                                           double* __restrict__ az, double* __restrict__ pot,
-                                                          DevScalars* sc) {
+                                                          DevScalars* sc, const float* __restrict__ amag,
+                                                          float rel2, float th2) {
   __shared__ int32_t stk_b[kPhysStack];  // donated ranges [stk_b, stk_e)
   __shared__ int32_t stk_e[kPhysStack];
   __shared__ int32_t bfirst[kWave];      // buckets of the current step: first body,
@@ -332,6 +333,17 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   const float hz = (float)(0.5 * (hi[2] - lo[2])) * (1.0f + 1e-6f);
   const T tx = (T)(px - gcx), ty = (T)(py - gcy), tz = (T)(pz - gcz);
   const T e2 = (T)(eps * eps);
+  // Relative criterion (amag != null): a cell that passes the geometric test
+  // is accepted only if its first neglected (octupole) term, ~ m bmax^3 / d^5,
+  // is at most alpha times the smallest previous |a| of the group:
+  // m^2 bmax^6 <= alpha^2 amin^2 d^10, with bmax^2 = rc2 * theta^2.
+  float arel = 0.f;
+  if (amag) {
+    float am = amag[ii];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) am = fminf(am, __shfl_xor(am, off, kWave));
+    arel = rel2 * am * am;
+  }
 
   double Ax = 0, Ay = 0, Az = 0, P = 0;
   uint32_t ncell_tot = 0, npart_tot = 0, niter = 0, nopen = 0;
@@ -421,9 +433,18 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       cnt = nd.count;
       first = nd.first;
       nskip = nd.skip > cur ? nd.skip : cur + 1;           // skip > cur by construction
-      if (d2 > nd.rc2) acc = true;
-      else if (cnt <= nleaf) bucket = true;
-      else donate = nskip < end;                           // open; the rest of the range can go
+      if (d2 > nd.rc2) {
+        acc = true;
+        if (amag) {
+          const float b2 = nd.rc2 * th2;
+          const float d4 = d2 * d2;
+          acc = nd.m * nd.m * (b2 * b2 * b2) <= arel * (d4 * d4 * d2);
+        }
+      }
+      if (!acc) {
+        if (cnt <= nleaf) bucket = true;
+        else donate = nskip < end;                         // open; the rest of the range can go
+      }
     }
     // accepted cells -> cell list
     const uint64_t am = __ballot(acc);
@@ -592,10 +613,10 @@ __global__ __launch_bounds__(kBlock) void k_energy_final(const double* __restric
 template <typename T>
 void walk_dispatch(unsigned nb, const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
                    const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
-                   DevScalars* sc, bool kick, bool count, hipStream_t s) {
+                   DevScalars* sc, bool kick, bool count, const float* amag, float rel2, float th2, hipStream_t s) {
 #define NBX_PW(K, P, C)                                                                                     \
   hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, groups, ngroups, pn, st, sy, \
-                     eps, nleaf, ax, ay, az, pot, sc)
+                     eps, nleaf, ax, ay, az, pot, sc, amag, rel2, th2)
   const bool p = pot != nullptr;
   if (kick) {
     if (count) NBX_PW(true, false, true); else NBX_PW(true, false, false);
@@ -651,13 +672,18 @@ void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const
 
 void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
                       const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
-                      DevScalars* sc, bool kick, bool count, bool fp64, hipStream_t s) {
+                      DevScalars* sc, bool kick, bool count, bool fp64, const float* amag, double alpha,
+                      double theta, hipStream_t s) {
   if (ngroups <= 0) return;
   const unsigned nb = (unsigned)(div_up(ngroups, 8) * 8);
+  const float rel2 = (float)(alpha * alpha), th2 = (float)(theta * theta);
+  if (alpha <= 0.0) amag = nullptr;
   if (fp64)
-    walk_dispatch<double>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
+    walk_dispatch<double>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
+                          th2, s);
   else
-    walk_dispatch<float>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, s);
+    walk_dispatch<float>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
+                         th2, s);
 }
 
 void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,

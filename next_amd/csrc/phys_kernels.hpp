@@ -66,10 +66,14 @@ void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const
                           uint32_t* ngroups, uint32_t* scratch, hipStream_t s);
 // Group walk over the group table: accelerations (and potentials if
 // pot != null), optionally followed by the closing half kick.  fp64 selects
-// the FP64 evaluation.
+// the FP64 evaluation.  With amag (|a| of each body from the previous
+// evaluation, FP32) and alpha > 0, a cell that passes the geometric test
+// (bmax / theta) is accepted only if m bmax^3 <= alpha |a|min d^5, |a|min
+// the smallest |a| of the group and d the distance to the group's box.
 void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
                       const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
-                      DevScalars* sc, bool kick, bool count, bool fp64, hipStream_t s);
+                      DevScalars* sc, bool kick, bool count, bool fp64, const float* amag, double alpha,
+                      double theta, hipStream_t s);
 // FP64 direct sum with Plummer softening over all pairs of each system (G = 1).
 void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,
                      double* az, double* pot, hipStream_t s);

@@ -707,20 +707,24 @@ __global__ __launch_bounds__(kBlock) void k_walk(const Group* __restrict__ group
 }
 
 // Opening half kick and drift (step.h:294-325), each system with its dt.
+// amag (may be null) receives |a| of each body, in FP32, for an opening
+// criterion relative to the previous acceleration.
 __global__ __launch_bounds__(kBlock) void k_kick_drift(int64_t n, StateArrays st, const double* __restrict__ ax,
                                                        const double* __restrict__ ay, const double* __restrict__ az,
-                                                       SysArrays sy) {
+                                                       SysArrays sy, float* __restrict__ amag) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const double dt = sy.dt[st.sys[i]];
   const double half = dt * 0.5;
-  const double vx = st.vx[i] + ax[i] * half;
-  const double vy = st.vy[i] + ay[i] * half;
-  const double vz = st.vz[i] + az[i] * half;
+  const double gx = ax[i], gy = ay[i], gz = az[i];
+  const double vx = st.vx[i] + gx * half;
+  const double vy = st.vy[i] + gy * half;
+  const double vz = st.vz[i] + gz * half;
   st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
   st.x[i] = st.x[i] + vx * dt;
   st.y[i] = st.y[i] + vy * dt;
   st.z[i] = st.z[i] + vz * dt;
+  if (amag) amag[i] = (float)sqrt(gx * gx + gy * gy + gz * gz);
 }
 
 // Max |v|^2 per system (adaptive.h:31-37).  Each thread takes kMaxv2Items
@@ -858,6 +862,11 @@ void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uin
 void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint32_t* dst, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_gather_sys, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, sys, dst);
 }
+void launch_gather_u32(int64_t n, const uint32_t* perm, const uint32_t* src, uint32_t* dst, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_gather_sys, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm,
+                       reinterpret_cast<const int32_t*>(src), dst);
+}
 void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
                          hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_gather_state, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
@@ -916,8 +925,8 @@ void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(n)), dim3(kBlock), 0, s, n, a, v);
 }
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
-                       const SysArrays& sy, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_kick_drift, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, ax, ay, az, sy);
+                       const SysArrays& sy, float* amag, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_kick_drift, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, ax, ay, az, sy, amag);
 }
 void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStream_t s) {
   if (n > 0)

@@ -35,6 +35,8 @@ void launch_keys2(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s);
 void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uint64_t* dst, hipStream_t s);
 void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint32_t* dst, hipStream_t s);
+// dst[i] = src[perm[i]] for 32-bit words.
+void launch_gather_u32(int64_t n, const uint32_t* perm, const uint32_t* src, uint32_t* dst, hipStream_t s);
 void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
                          hipStream_t s);
 void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, const int32_t* sys, int32_t* lcp,
@@ -54,8 +56,9 @@ void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s);
 void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
                  const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
                  bool count, hipStream_t s);
+// amag (may be null): |a| per body in FP32.
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
-                       const SysArrays& sy, hipStream_t s);
+                       const SysArrays& sy, float* amag, hipStream_t s);
 void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStream_t s);
 void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, hipStream_t s);
 void launch_export(int64_t total, const RefNode* nodes, const uint64_t* key, const uint64_t* key2,

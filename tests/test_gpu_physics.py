@@ -108,6 +108,45 @@ def test_physics_mode_rejects_bad_settings():
             phys(**args)
 
 
+def test_relative_opening_criterion():
+    """alpha > 0: after the first (geometric) evaluation, cells that pass the
+    bmax/theta test must also satisfy m bmax^3 <= alpha |a|min d^5.  On a
+    Plummer sphere with a weak geometric guard (theta 0.8) and alpha 1e-3 the
+    force error stays at the level of the geometric theta 0.4 walk (measured
+    median 1.4e-4 and p99 5.6e-4 on the 512-system ensemble) with fewer
+    interactions; alpha = 0 leaves the geometric walk unchanged bit for bit."""
+    p = ic.plummer(20000, seed=31, virial=None)
+    counts = {}
+    for alpha, theta in ((0.0, 0.8), (1e-3, 0.8)):
+        with phys(theta=theta) as e:
+            e.set_alpha(alpha)
+            e.upload(p)
+            e.run(1e-3, 1)
+            a = e.accel()
+            ad, _ = e.direct()
+            err = rel_err(a, ad)
+            e.set_profiling(2)
+            e.reset_timing()
+            e.run(1e-3, 1)
+            t = e.timing()
+            counts[alpha] = t["n_pp"] + t["n_pc"]
+        if alpha > 0:
+            assert np.median(err) <= 3e-4 and np.percentile(err, 99) <= 2e-3
+        else:
+            with phys(theta=theta) as e:
+                e.upload(p)
+                e.run(1e-3, 1)
+                assert np.array_equal(e.accel(), a)
+    assert counts[1e-3] > counts[0.0]           # the relative test only ever rejects more cells
+    with phys() as e:
+        for bad in (-1.0, float("inf"), float("nan")):
+            with pytest.raises(NbxError):
+                e.set_alpha(bad)
+    with Engine(theta=0.5) as e:                # reference mode: NEXT's geometric criterion only
+        with pytest.raises(NbxError):
+            e.set_alpha(1e-3)
+
+
 def subtree_ranges(depth, leaf):
     """Body range [lo, hi) in leaf order of every node of a preorder table."""
     n_nodes = len(depth)
