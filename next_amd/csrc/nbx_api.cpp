@@ -111,12 +111,11 @@ nbx_status fail(nbx_ctx* c, nbx_status s, const std::string& msg) {
     if (s_ != NBX_OK) return s_;              \
   } while (0)
 
-// Grow a buffer to at least `bytes`; `keep` bytes of the old contents are
-// copied over (0: contents not kept).
-// Grow a device buffer to at least `bytes` (keeping its first `keep` bytes).
-// A buffer that has to grow gets 1/8 extra: tree sizes change a little
-// from step to step, and every reallocation (hipMalloc/hipFree) stalls the
-// device for hundreds of microseconds.
+// Grow a device buffer to at least `bytes`; its first `keep` bytes are
+// copied over (0: contents not kept).  A buffer that has to grow gets 1/8
+// extra: tree sizes change a little from step to step, and every
+// reallocation (hipMalloc/hipFree) stalls the device for hundreds of
+// microseconds.
 nbx_status ensure(nbx_ctx* c, DBuf& b, size_t bytes, size_t keep = 0) {
   if (bytes <= b.bytes && b.p) return NBX_OK;
   const size_t want = std::max<size_t>(b.p ? bytes + bytes / 8 : bytes, 256);
