@@ -39,6 +39,7 @@ from next_amd import ic  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP, FLOPS_PC = 19, 55
+REBUILD_EVERY = 4         # ensembles: full tree build every 4th step, refits between
 
 
 def log(msg: str) -> None:
@@ -85,15 +86,17 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
     samples of the distribution function, not rescaled to an exact virial
     ratio), N_i log-uniform in [1024, 65536] from seed i, theta = 0.5,
     dt = 1e-3, FP32 walk with FP64 state, all advanced by one set of launches
-    per step; leaf and group size chosen by the autotuner first."""
+    per step, the tree rebuilt every REBUILD_EVERY steps and refitted between;
+    leaf and group size chosen by the autotuner first."""
     from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
 
     sizes = ic.ensemble_sizes(nsys)
     systems = [ic.plummer(n, seed=i, virial=None) for i, n in enumerate(sizes)]
     cands = [(leaf, group) for leaf in (8, 16, 32) for group in (32, 64)]
-    with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False) as e:
+    with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False,
+                rebuild_every=REBUILD_EVERY) as e:
         e.upload_batch(systems)
-        best, trial_ms = e.autotune(cands, 1e-3, steps=2)   # leaves the state as it was
+        best, trial_ms = e.autotune(cands, 1e-3, steps=4)   # leaves the state as it was
         e.run(1e-3, warmup)
         t0 = time.perf_counter()
         e.run(1e-3, steps)
@@ -104,6 +107,7 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
             "ms_per_step": dt * 1e3 / steps, "body_steps_per_s": total * steps / dt,
             "autotuned": {"leaf_size": best[0], "group_size": best[1],
                           "trial_ms_per_step": {f"{lf}/{g}": round(float(m), 3) for (lf, g), m in zip(cands, trial_ms)}},
+            "tree_rebuild_every": REBUILD_EVERY,
             "workload": f"{nsys} Plummer spheres (DF samples), N log-uniform [1024, 65536], theta 0.5 (bmax criterion), "
                         "eps 0.01, dt 1e-3, FP32 walk / FP64 state",
             "at_target_accuracy": target}
@@ -120,7 +124,7 @@ def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, ener
     from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
 
     with Engine(mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, theta=theta, leaf_size=best[0],
-                group_size=best[1], adaptive_dt=False, alpha=alpha) as e:
+                group_size=best[1], adaptive_dt=False, alpha=alpha, rebuild_every=REBUILD_EVERY) as e:
         e.upload_batch(systems)
         e.run(1e-3, 1)                 # the criterion needs the previous accelerations
         a = e.accel()
@@ -370,7 +374,7 @@ def main() -> int:
     log("headline: %.3f ms/step" % ms_per_step)
     if not args.no_extra:
         out["physics"] = physics(args.n, ms_per_step)
-        out["ensemble"] = ensemble(args.ensemble_systems, steps=5, warmup=1)
+        out["ensemble"] = ensemble(args.ensemble_systems, steps=8, warmup=1)
         log("ensemble: %.1f ms/step" % out["ensemble"]["ms_per_step"])
         out["serving"] = []
         for rate in (50.0, 200.0):

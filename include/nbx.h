@@ -114,6 +114,15 @@ nbx_status nbx_set_theta(nbx_ctx* ctx, double theta);
  * turns it off.  NBX_ERR_ARG for alpha < 0, non-finite, or reference mode. */
 nbx_status nbx_set_alpha(nbx_ctx* ctx, double alpha);
 
+/* Physics mode: rebuild the tree (keys, sort, layout, walk groups) only on
+ * every k-th step and refit it in between: the bodies keep their order and
+ * the tree its topology, while leaf records, centres of mass, quadrupoles
+ * and bmax are recomputed from the current positions, so the opening test
+ * stays conservative (no counterpart in NEXT, which rebuilds every step).
+ * k = 1 (the default) rebuilds every step.  NBX_ERR_ARG for k outside
+ * 1..1000, or k > 1 in reference mode. */
+nbx_status nbx_set_rebuild_interval(nbx_ctx* ctx, int32_t k);
+
 /* Copy n particles in (Particle SoA of particle.h:22-52).  n may be 0. */
 nbx_status nbx_upload(nbx_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                       const double* vx, const double* vy, const double* vz, const double* m,

@@ -79,6 +79,12 @@ struct nbx_ctx {
   bool amag_live = false;     // amag[acur] follows the bodies through this step's sort
   bool acc_valid = false;     // acc holds a(x) of the current positions
   bool tree_valid = false;    // nodes hold the tree of the current positions
+  // Physics mode: the body order, node layout and walk groups of the last
+  // build still describe the bodies (positions may have moved since), so
+  // the tree can be refitted instead of rebuilt.
+  bool layout_valid = false;
+  int32_t rebuild_every = 1;  // physics mode: full build every k-th step, refits between
+  int32_t since_build = 0;    // steps since the last full build
   const uint64_t* skey = nullptr;
   const uint64_t* skey2 = nullptr;
   int64_t total_nodes = 0;
@@ -230,6 +236,7 @@ nbx_status set_systems(nbx_ctx* c) {
   c->nboxes = (int)boxes.size();
   c->acc_valid = false;
   c->tree_valid = false;
+  c->layout_valid = false;
   return NBX_OK;
 }
 
@@ -421,6 +428,7 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     launch_group_compact(n, gcnt, gmark, St.sys, P<Group>(c->tgroups), &c->sc->tree_groups, scr, s);
     NBXCHK(sync_scalars(c));
     c->ntgroups = c->hsc->tree_groups;
+    c->layout_valid = true;
   } else {
     NBXCHK(ref_layout(c));
     record(c, EV_BUILD);
@@ -429,6 +437,36 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
       if (c->hsc->level_count[d] > 0)
         launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, c->cfg.theta, s);
   }
+  record(c, EV_UPWARD);
+  c->tree_valid = true;
+  return NBX_OK;
+}
+
+// Physics mode: moments of the existing tree for the current positions.
+// The bodies keep their order, so the node layout, skip pointers and walk
+// groups of the last build still hold; leaf records, centres of mass,
+// quadrupoles and bmax are recomputed from the bodies (bmax from their
+// actual box), so every opening decision stays conservative however far
+// the bodies have moved.  The group marks the passes write are not used.
+nbx_status refit_tree(nbx_ctx* c) {
+  const int64_t n = c->n;
+  hipStream_t s = c->stream;
+  record(c, EV_KEYS);
+  record(c, EV_SORT);
+  int32_t* gcnt = P<int32_t>(c->gmarks);
+  uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
+  PhysNode* pn = P<PhysNode>(c->pnodes);
+  launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), c->sc, pn,
+                   P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
+                   gcnt, gmark, s);
+  record(c, EV_BUILD);
+  int64_t off[kMaxDepthSlots + 1];
+  off[0] = 0;
+  for (int d = 0; d < kMaxDepthSlots; ++d) off[d + 1] = off[d] + c->hsc->level_count[d];
+  for (int d = kMaxDepthSlots - 1; d >= 0; --d)
+    if (c->hsc->level_count[d] > 0)
+      launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], d, pn, c->paux.p, c->cfg.theta,
+                         c->cfg.group_size, gcnt, gmark, s);
   record(c, EV_UPWARD);
   c->tree_valid = true;
   return NBX_OK;
@@ -517,7 +555,9 @@ nbx_status step_device(nbx_ctx* c) {
   c->tree_valid = false;
   c->amag_live = rel;
   record(c, EV_KICK);
-  const nbx_status st = build_tree(c);
+  const bool refit = c->physics && c->layout_valid && c->since_build + 1 < c->rebuild_every;
+  const nbx_status st = refit ? refit_tree(c) : build_tree(c);
+  c->since_build = refit ? c->since_build + 1 : 0;
   if (st != NBX_OK) { c->amag_live = false; return st; }
   const nbx_status sw = walk(c, true);
   c->amag_live = false;
@@ -696,6 +736,7 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
     c->cur = cur0;
     c->acc_valid = false;
     c->tree_valid = false;
+    c->layout_valid = false;
     return NBX_OK;
   };
   const nbx_config keep = c->cfg;
@@ -705,6 +746,7 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
   for (int32_t k = 0; k < ncand; ++k) {
     c->cfg.leaf_size = leaf[k];
     c->cfg.group_size = group[k];
+    c->layout_valid = false;                               // groups of these sizes need a full build
     nbx_status st = nbx_run(c, dt, 1, nullptr);            // warm-up: buffers, first tree
     const auto t0 = std::chrono::steady_clock::now();
     if (st == NBX_OK) st = nbx_run(c, dt, steps, nullptr);
@@ -735,10 +777,20 @@ nbx_status nbx_set_alpha(nbx_ctx* c, double alpha) {
   return NBX_OK;
 }
 
+nbx_status nbx_set_rebuild_interval(nbx_ctx* c, int32_t every) {
+  if (!c) return NBX_ERR_ARG;
+  if (every < 1 || every > 1000) return fail(c, NBX_ERR_ARG, "the rebuild interval must be 1..1000 steps");
+  if (every > 1 && !c->physics) return fail(c, NBX_ERR_ARG, "tree refits are a physics-mode option");
+  c->rebuild_every = every;
+  c->since_build = 0;
+  return NBX_OK;
+}
+
 nbx_status nbx_set_theta(nbx_ctx* c, double theta) {
   if (!c || !(theta >= 0.0)) return NBX_ERR_ARG;
   if (c->physics && !(theta > 0.0 && theta < 1.0)) return fail(c, NBX_ERR_ARG, "physics mode needs 0 < theta < 1");
   c->tree_valid = false;
+  c->layout_valid = false;
   c->cfg.theta = theta;
   c->acc_valid = false;
   return NBX_OK;
@@ -1038,6 +1090,7 @@ nbx_status nbx_debug_keys(nbx_ctx* c, double* root4, uint64_t* hi, uint64_t* lo)
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->skey2 = nullptr;  // the second-word buffers were reused
   c->tree_valid = false;
+  c->layout_valid = false;
   return NBX_OK;
 }
 

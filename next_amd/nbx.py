@@ -23,6 +23,7 @@ MODE_REFERENCE, MODE_PHYSICS = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_set_alpha",
+           "nbx_set_rebuild_interval",
            "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
            "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
@@ -90,6 +91,7 @@ def lib():
     L.nbx_last_error.restype = C.c_char_p
     L.nbx_set_theta.argtypes = [VP, C.c_double]
     L.nbx_set_alpha.argtypes = [VP, C.c_double]
+    L.nbx_set_rebuild_interval.argtypes = [VP, C.c_int32]
     L.nbx_upload.argtypes = [VP, C.c_int64] + [f64p] * 7 + [i32p]
     L.nbx_download.argtypes = [VP] + [f64p] * 6
     L.nbx_size.argtypes = [VP]
@@ -139,7 +141,7 @@ class Engine:
 
     def __init__(self, theta: float = 0.5, device: int = 0, mode: int = MODE_REFERENCE,
                  precision: int = PREC_FP64, eps: float = 0.01, leaf_size: int = 16, group_size: int = 64,
-                 adaptive_dt: bool = True, alpha: float = 0.0):
+                 adaptive_dt: bool = True, alpha: float = 0.0, rebuild_every: int = 1):
         L = lib()
         cfg = Config()
         L.nbx_config_default(C.byref(cfg))
@@ -154,6 +156,8 @@ class Engine:
         self.n = 0
         if alpha:
             self.set_alpha(alpha)
+        if rebuild_every != 1:
+            self.set_rebuild_interval(rebuild_every)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -179,6 +183,10 @@ class Engine:
     def set_alpha(self, alpha: float):
         """Physics mode: relative opening criterion (0 turns it off)."""
         self._check(lib().nbx_set_alpha(self._h, alpha))
+
+    def set_rebuild_interval(self, k: int):
+        """Physics mode: full tree build every k-th step, refits between."""
+        self._check(lib().nbx_set_rebuild_interval(self._h, k))
 
     def upload(self, p: Particles):
         n = len(p)

@@ -147,6 +147,36 @@ def test_relative_opening_criterion():
             e.set_alpha(1e-3)
 
 
+def test_tree_refit_between_rebuilds():
+    """rebuild_every = 4: three of every four steps refit the tree (same body
+    order and topology, moments and bmax from the current positions).  The
+    force error against the FP64 direct sum stays that of a rebuild every
+    step, and the trajectories agree to the level of the force error."""
+    p = ic.plummer(20000, seed=33, virial=None)
+    out = {}
+    for k in (1, 4):
+        with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False,
+                    rebuild_every=k) as e:
+            e.upload(p)
+            e.run(1e-3, 7)                         # ends on a refit for k = 4
+            a = e.accel()
+            ad, _ = e.direct()
+            out[k] = (np.median(rel_err(a, ad)), np.stack(e.download()))
+    assert out[4][0] <= 1.2 * out[1][0]
+    # forces differ at the multipole-error level (~1e-4 of |a|, |a| up to ~10
+    # in the core) over 7 steps of 1e-3: velocities by up to ~1e-5, positions
+    # by up to ~1e-7
+    d = np.abs(out[4][1] - out[1][1])
+    assert d[:3].max() <= 5e-7 and d[3:].max() <= 5e-5
+    with phys() as e:
+        for bad in (0, 1001):
+            with pytest.raises(NbxError):
+                e.set_rebuild_interval(bad)
+    with Engine(theta=0.5) as e:                    # reference mode rebuilds every step, as NEXT does
+        with pytest.raises(NbxError):
+            e.set_rebuild_interval(2)
+
+
 def subtree_ranges(depth, leaf):
     """Body range [lo, hi) in leaf order of every node of a preorder table."""
     n_nodes = len(depth)
