@@ -10,14 +10,18 @@
 // Multipoles (FP64): m, centre of mass, Q_ij = sum m (3 r_i r_j - r^2 d_ij)
 // about the centre of mass, propagated with the parallel-axis rule
 //   Q = sum_c [Q_c + m_c (3 r_c r_c^T - |r_c|^2 I)],  r_c = com_c - com,
-// and bmax, a bound on the distance from the centre of mass to any body:
-//   bmax = max_c (|r_c| + bmax_c).
+// and bmax, a bound on the distance from the centre of mass to any body: the
+// smaller of the distance to the farthest corner of the bodies' bounding box
+// and max_c (|r_c| + bmax_c).  A refit (bodies moved, same order and
+// topology) recomputes all of these from the current positions.
 //
 // Walk: one wave = one group of at most 64 targets packed from the tree
 // (consecutive sibling subtrees inside one node).  The wave walks the
 // preorder tree with a decision shared by the whole group: a node is
 // accepted when the distance from its centre of mass to the group's bounding
-// box exceeds bmax/theta.  Lanes walk disjoint ranges of the preorder array
+// box exceeds bmax/theta (and, with the relative criterion, when its
+// octupole-order term is small against the group's previous |a|).  Lanes
+// walk disjoint ranges of the preorder array
 // and hand work to idle lanes (k_phys_walk).  Accepted cells and bucket
 // bodies are appended, relative to the group centre, to per-wave LDS lists;
 // a full list is evaluated by all the group's lanes at once (entries read as
