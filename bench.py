@@ -207,21 +207,24 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
             "by_theta": rows, "energy": energy}
 
 
-def serving(nreq: int, rate: float, seed: int = 5) -> dict:
+def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(20, 60)) -> dict:
     """Serving sample: Poisson arrivals at `rate` per second of independent
-    Plummer or cold-sphere runs, N log-uniform in [4096, 131072], 20-60 steps,
-    theta from {0.4, 0.5, 0.7}, FP32 or FP64 walk, 200 GB budget."""
+    Plummer or cold-sphere runs, N log-uniform in [4096, n_max], step counts
+    uniform in `steps`, theta from {0.4, 0.5, 0.7}, FP32 or FP64 walk, 200 GB
+    budget."""
     from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Server
 
     rng = np.random.Generator(np.random.PCG64(seed))
     reqs = []
     for k in range(nreq):
-        n = int(round(np.exp(rng.uniform(np.log(4096), np.log(131072)))))
+        n = int(round(np.exp(rng.uniform(np.log(4096), np.log(n_max)))))
         p = (ic.plummer(n, seed=1000 + k, virial=None) if rng.uniform() < 0.5
              else ic.uniform_sphere(n, seed=1000 + k))
-        reqs.append(dict(p=p, nsteps=int(rng.integers(20, 61)), theta=float(rng.choice([0.4, 0.5, 0.7])),
+        reqs.append(dict(p=p, nsteps=int(rng.integers(steps[0], steps[1] + 1)), theta=float(rng.choice([0.4, 0.5, 0.7])),
                          precision=PREC_MIXED if rng.uniform() < 0.5 else PREC_FP64,
                          gap=float(rng.exponential(1.0 / rate))))
+        if (k + 1) % 50 == 0:
+            log("serving at %g/s: %d of %d requests generated" % (rate, k + 1, nreq))
     lat, ids = [], []
     with Server(budget_bytes=200 << 30) as srv:
         t0 = time.perf_counter()
@@ -229,16 +232,19 @@ def serving(nreq: int, rate: float, seed: int = 5) -> dict:
             time.sleep(r["gap"])
             ids.append(srv.submit(r["p"], r["nsteps"], 1e-3, mode=MODE_PHYSICS, precision=r["precision"],
                                   theta=r["theta"], eps=0.01, leaf_size=16))
-        for i in ids:
+        for k, i in enumerate(ids):
             _, _, l_ms = srv.wait(i)
             lat.append(l_ms)
+            if (k + 1) % 25 == 0:
+                log("serving at %g/s: %d of %d requests done" % (rate, k + 1, nreq))
         wall = time.perf_counter() - t0
         st = srv.stats()
     return {"requests": nreq, "arrival_rate_per_s": rate, "p50_latency_ms": float(np.percentile(lat, 50)),
             "p99_latency_ms": float(np.percentile(lat, 99)), "body_steps_per_s": st["body_steps"] / wall,
             "batches": st["batches"],
-            "workload": "Poisson arrivals; Plummer (DF sample) or cold sphere, N log-uniform [4096, 131072], 20-60 steps, "
-                        "theta {0.4, 0.5, 0.7}, FP32/FP64 walk, dt 1e-3, eps 0.01, 200 GB budget"}
+            "workload": f"Poisson arrivals; Plummer (DF sample) or cold sphere, N log-uniform [4096, {n_max}], "
+                        f"{steps[0]}-{steps[1]} steps, theta {{0.4, 0.5, 0.7}}, FP32/FP64 walk, dt 1e-3, eps 0.01, "
+                        "200 GB budget"}
 
 
 # Algorithmic HBM bytes per body of the streaming phases (state: x y z vx vy
@@ -285,6 +291,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and serving samples")
     ap.add_argument("--ensemble-systems", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-config", action="store_true",
+                    help="serving sample with the configuration's distributions (N up to 2^20, 50-500 steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB device buffer 3 times (scale for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -377,8 +385,9 @@ def main() -> int:
         out["ensemble"] = ensemble(args.ensemble_systems, steps=8, warmup=1)
         log("ensemble: %.1f ms/step" % out["ensemble"]["ms_per_step"])
         out["serving"] = []
+        full = dict(n_max=1 << 20, steps=(50, 500)) if args.serve_config else {}
         for rate in (50.0, 200.0):
-            out["serving"].append(serving(args.serve_requests, rate))
+            out["serving"].append(serving(args.serve_requests, rate, **full))
             log("serving at %g/s: p50 %.1f ms" % (rate, out["serving"][-1]["p50_latency_ms"]))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.base_dt, args.theta, args.cpu_steps)
