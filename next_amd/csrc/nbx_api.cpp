@@ -91,6 +91,11 @@ struct nbx_ctx {
   int32_t max_depth = 0;
   int prof = 0;
   hipEvent_t ev[EV_COUNT] = {};
+  hipEvent_t ev_scalars = nullptr;  // the build's scalar read-back has landed
+  // The layout kernel ran before the host knew the node counts, bounded by
+  // these capacities; valid if the tree fits them.
+  bool spec_emit = false;
+  uint32_t spec_nodes = 0, spec_int = 0;
   nbx_timing tim{};
 };
 
@@ -148,6 +153,14 @@ void release(DBuf& b) {
 }
 
 size_t elem_bytes(int k) { return k < 7 ? 8 : 4; }
+
+uint32_t cap_words(const DBuf& b, size_t esz) { return (uint32_t)std::min<size_t>(b.bytes / esz, UINT32_MAX); }
+
+// Entries the four depth-sort key arrays can hold.
+uint32_t cap_int(const nbx_ctx* c) {
+  return std::min({cap_words(c->st_key, 4), cap_words(c->st_val, 4), cap_words(c->en_key, 4),
+                   cap_words(c->en_val, 4)});
+}
 
 StateArrays view(nbx_ctx* c, int which) {
   DBuf* s = c->st[which];
@@ -294,14 +307,39 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
   scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
   HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipEventRecord(c->ev_scalars, s));
+  // Lay out the nodes now, into the buffers of the last build, while the
+  // host waits for the node counts; build_post keeps the result if the tree
+  // fits those buffers (the kernel writes nothing beyond them).
+  c->spec_emit = false;
+  const uint32_t ci = cap_int(c);
+  if (c->physics && c->pnodes.p && ci > 0) {
+    NBXCHK(ensure(c, c->gmarks, (size_t)n * 8));
+    launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
+    c->spec_nodes = cap_words(c->pnodes, sizeof(PhysNode));
+    c->spec_int = ci;
+    launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, c->sc,
+                     P<PhysNode>(c->pnodes), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
+                     P<uint32_t>(c->en_val), P<int32_t>(c->gmarks), P<uint32_t>(c->gmarks) + n, c->spec_nodes,
+                     c->spec_int, s);
+    c->spec_emit = true;
+  } else if (!c->physics && c->nodes.p && ci > 0) {
+    c->spec_nodes = cap_words(c->nodes, sizeof(RefNode));
+    c->spec_int = ci;
+    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, sy, c->sc,
+                P<RefNode>(c->nodes), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
+                P<uint32_t>(c->en_val), c->spec_nodes, c->spec_int, s);
+    c->spec_emit = true;
+  }
   *skey_out = skey;
   return NBX_OK;
 }
 
 // Reference node records of the current tree (layout and skip pointers;
-// the moments come from the upward pass): k_emit, the two depth sorts and
-// k_pair.  Needs the scalars of the last build on the host.
-nbx_status ref_layout(nbx_ctx* c) {
+// the moments come from the upward pass): k_emit (unless `emitted`: already
+// run into buffers that hold the tree), the two depth sorts and k_pair.
+// Needs the scalars of the last build on the host.
+nbx_status ref_layout(nbx_ctx* c, bool emitted) {
   const int64_t n = c->n, total = c->total_nodes;
   const int64_t m = (uint32_t)c->hsc->internal_nodes;
   hipStream_t s = c->stream;
@@ -310,9 +348,10 @@ nbx_status ref_layout(nbx_ctx* c) {
   NBXCHK(ensure(c, c->nodes, (size_t)(total + 1) * sizeof(RefNode)));
   uint32_t* scr = P<uint32_t>(c->scratch);
   RefNode* nodes = P<RefNode>(c->nodes);
-  launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), sysa(c), c->sc,
-              nodes, P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
-              s);
+  if (!emitted)
+    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), sysa(c),
+                c->sc, nodes, P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
+                P<uint32_t>(c->en_val), cap_words(c->nodes, sizeof(RefNode)), cap_int(c), s);
   const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
                                  P<uint32_t>(c->st_val2), m, 8, scr, s);
   const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
@@ -333,11 +372,12 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   uint32_t* scr = P<uint32_t>(c->scratch);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
   uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
-  HIPCHK(c, hipStreamSynchronize(s));
+  HIPCHK(c, hipEventSynchronize(c->ev_scalars));
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
 
   const uint64_t* skey2 = nullptr;
   if (c->hsc->collisions > 0) {
+    c->spec_emit = false;                 // laid out from the first key word alone
     // Bodies of one system sharing all 21 digits of the first key word: sort
     // by (system, first word, second word), least significant first.
     for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)c->cap * 8));
@@ -410,10 +450,13 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     int32_t* gcnt = P<int32_t>(c->gmarks);
     uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
     PhysNode* pn = P<PhysNode>(c->pnodes);
-    launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
-    launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, c->sc, pn,
-                     P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
-                     gcnt, gmark, s);
+    const bool emitted = c->spec_emit && total <= (int64_t)c->spec_nodes && m <= (int64_t)c->spec_int;
+    if (!emitted) {
+      launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
+      launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, c->sc, pn,
+                       P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
+                       P<uint32_t>(c->en_val), gcnt, gmark, cap_words(c->pnodes, sizeof(PhysNode)), cap_int(c), s);
+    }
     const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
                                    P<uint32_t>(c->st_val2), m, 8, scr, s);
     const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
@@ -430,7 +473,7 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     c->ntgroups = c->hsc->tree_groups;
     c->layout_valid = true;
   } else {
-    NBXCHK(ref_layout(c));
+    NBXCHK(ref_layout(c, c->spec_emit && total + 1 <= (int64_t)c->spec_nodes && m <= (int64_t)c->spec_int));
     record(c, EV_BUILD);
     RefNode* nodes = P<RefNode>(c->nodes);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
@@ -458,7 +501,7 @@ nbx_status refit_tree(nbx_ctx* c) {
   PhysNode* pn = P<PhysNode>(c->pnodes);
   launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), c->sc, pn,
                    P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
-                   gcnt, gmark, s);
+                   gcnt, gmark, cap_words(c->pnodes, sizeof(PhysNode)), cap_int(c), s);
   record(c, EV_BUILD);
   int64_t off[kMaxDepthSlots + 1];
   off[0] = 0;
@@ -673,6 +716,10 @@ nbx_status nbx_create(nbx_ctx** out, const nbx_config* cfg) {
   }
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) { nbx_destroy(c); return NBX_ERR_HIP; }
+  if (hipEventCreateWithFlags(&c->ev_scalars, hipEventDisableTiming) != hipSuccess) {
+    nbx_destroy(c);
+    return NBX_ERR_HIP;
+  }
   std::memset(c->hsc, 0, sizeof(DevScalars));
   if (hipMemcpyAsync(c->sc, c->hsc, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess || set_systems(c) != NBX_OK) {
@@ -699,6 +746,7 @@ void nbx_destroy(nbx_ctx* c) {
                   &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part, &c->tgroups, &c->gmarks})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->ev_scalars) (void)hipEventDestroy(c->ev_scalars);
   if (c->sc) (void)hipFree(c->sc);
   if (c->hsc) (void)hipHostFree(c->hsc);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1143,7 +1191,7 @@ nbx_status nbx_debug_tree(nbx_ctx* c, int32_t* depth, int32_t* leaf, int32_t* bo
   if (c->physics) {
     // The physics build writes walk records only: lay out the reference
     // records of the same tree, with NEXT's moments.
-    NBXCHK(ref_layout(c));
+    NBXCHK(ref_layout(c, false));
     int64_t off = 0;
     for (int d = 0; d < kMaxDepthSlots; ++d) {
       const int64_t cnt = c->hsc->level_count[d];

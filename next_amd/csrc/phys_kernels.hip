@@ -70,7 +70,8 @@ __global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* 
                                                       const DevScalars* __restrict__ sc, PhysNode* __restrict__ pn,
                                                       uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
                                                       uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val,
-                                                      int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
+                                                      int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark,
+                                                      uint32_t cap_nodes, uint32_t cap_int) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int li = lcp[i];
@@ -80,11 +81,13 @@ __global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* 
   const uint32_t sbase = pre - (uint32_t)i;
   for (int t = 0; t < ns; ++t) {
     const uint32_t p = pre + t;
-    st_key[sbase + t] = (uint32_t)(lp + 1 + t);
-    st_val[sbase + t] = p;
-    pn[p].first = (int32_t)i;
+    if (sbase + t < cap_int) {
+      st_key[sbase + t] = (uint32_t)(lp + 1 + t);
+      st_val[sbase + t] = p;
+    }
+    if (p < cap_nodes) pn[p].first = (int32_t)i;
   }
-  {
+  if (pre + ns < cap_nodes) {
     const uint32_t p = pre + ns;
     PhysNode nd;
     nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* 
   if (ne) {
     const uint32_t skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
     const uint32_t eo = end_off[i];
-    for (int t = 0; t < ne; ++t) {
+    for (int t = 0; t < ne && eo + t < cap_int; ++t) {
       en_key[eo + t] = (uint32_t)(li + 1 + t);
       en_val[eo + t] = skipv;
     }
@@ -648,10 +651,11 @@ __global__ __launch_bounds__(kBlock) void k_group_compact(int64_t n, const int32
 
 void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
                       const StateArrays& st, const DevScalars* sc, PhysNode* pn, uint32_t* st_key, uint32_t* st_val,
-                      uint32_t* en_key, uint32_t* en_val, int32_t* gcnt, uint32_t* gmark, hipStream_t s) {
+                      uint32_t* en_key, uint32_t* en_val, int32_t* gcnt, uint32_t* gmark, uint32_t cap_nodes,
+                      uint32_t cap_int, hipStream_t s) {
   if (n > 0)
     hipLaunchKernelGGL(k_phys_emit, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sc,
-                       pn, st_key, st_val, en_key, en_val, gcnt, gmark);
+                       pn, st_key, st_val, en_key, en_val, gcnt, gmark, cap_nodes, cap_int);
 }
 
 void launch_phys_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, PhysNode* pn, int32_t* level_list,

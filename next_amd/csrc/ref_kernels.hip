@@ -326,7 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
                                                  SysArrays sy, const DevScalars* __restrict__ sc,
                                                  RefNode* __restrict__ nodes,
                                                  uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
-                                                 uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val) {
+                                                 uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val,
+                                                 uint32_t cap_nodes, uint32_t cap_int) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int li = lcp[i];
@@ -338,8 +339,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
   for (int t = 0; t < ns; ++t) {
     const int d = lp + 1 + t;
     const uint32_t p = pre + t;
-    st_key[sbase + t] = (uint32_t)d;
-    st_val[sbase + t] = p;
+    if (sbase + t < cap_int) {
+      st_key[sbase + t] = (uint32_t)d;
+      st_val[sbase + t] = p;
+    }
+    if (p >= cap_nodes) continue;
     // mass, centre of mass, softening numerator and quadrupole come from
     // k_upward, the skip pointer from k_pair
     RefNode& nd = nodes[p];
@@ -348,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
     nd.depth = d;
     nd.first = (int32_t)i;
   }
-  {
+  if (pre + ns < cap_nodes) {
     const uint32_t p = pre + ns;
     const int d = 1 + (li > lp ? li : lp);
     // Leaves carry no quadrupole; the walk never reads a leaf's Q fields.
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
   if (ne) {
     const uint32_t skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
     const uint32_t eo = end_off[i];
-    for (int t = 0; t < ne; ++t) {
+    for (int t = 0; t < ne && eo + t < cap_int; ++t) {
       en_key[eo + t] = (uint32_t)(li + 1 + t);
       en_val[eo + t] = skipv;
     }
@@ -897,10 +901,11 @@ void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevSca
 }
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
                  const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
-                 uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, hipStream_t s) {
+                 uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes, uint32_t cap_int,
+                 hipStream_t s) {
   if (n > 0)
     hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sy, sc, nodes,
-                       st_key, st_val, en_key, en_val);
+                       st_key, st_val, en_key, en_val, cap_nodes, cap_int);
 }
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
                  hipStream_t s) {
