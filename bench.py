@@ -191,9 +191,16 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
             t0 = time.perf_counter()
             e.run(5e-4, 10)
             ms = (time.perf_counter() - t0) * 1e2
+            # the same with the tree rebuilt every REBUILD_EVERY steps and refitted between
+            e.set_rebuild_interval(REBUILD_EVERY)
+            t0 = time.perf_counter()
+            e.run(5e-4, 3 * REBUILD_EVERY)
+            ms_refit = (time.perf_counter() - t0) * 1e3 / (3 * REBUILD_EVERY)
+            e.set_rebuild_interval(1)
             rows.append({"theta": theta, "ms_per_step": round(ms, 4), "median_rel_force_err": float(np.median(err)),
                          "p99_rel_force_err": float(np.percentile(err, 99)), "body_steps_per_s": n / (ms * 1e-3),
-                         "vs_reference_path": round(ref_ms / ms, 3)})
+                         "vs_reference_path": round(ref_ms / ms, 3),
+                         "ms_per_step_tree_rebuilt_every_%d" % REBUILD_EVERY: round(ms_refit, 4)})
             if j == 0 and energy_steps > 0:
                 e.upload(p)
                 e0 = e.energy(direct=True)
