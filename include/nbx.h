@@ -58,7 +58,8 @@ typedef enum nbx_mode {
 
 typedef enum nbx_precision {
   NBX_PREC_FP64 = 0,   /* FP64 walk */
-  NBX_PREC_MIXED = 1   /* FP32 walk and accumulation; FP64 multipoles and state */
+  NBX_PREC_MIXED = 1   /* FP32 walk (each list flush summed in FP32, added to FP64
+                          totals); FP64 multipoles and state */
 } nbx_precision;
 
 typedef struct nbx_config {
@@ -68,7 +69,8 @@ typedef struct nbx_config {
   double theta;      /* opening angle; NEXT fixes 0.5 (step.h:211) */
   double eps;        /* Plummer softening length (NBX_MODE_PHYSICS only) */
   int leaf_size;     /* max bodies per leaf bucket, 1..64 (NBX_MODE_PHYSICS only) */
-  int group_size;    /* targets per walk group; 64 (NBX_MODE_PHYSICS only) */
+  int group_size;    /* at most this many targets per walk group, 1..64 (NBX_MODE_PHYSICS
+                        only; reference mode walks runs of 64) */
   int adaptive_dt;   /* nbx_run: 1 = NEXT's speed-adaptive dt (adaptive.h), 0 = fixed dt */
 } nbx_config;
 
