@@ -199,7 +199,7 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
             e.set_rebuild_interval(1)
             rows.append({"theta": theta, "ms_per_step": round(ms, 4), "median_rel_force_err": float(np.median(err)),
                          "p99_rel_force_err": float(np.percentile(err, 99)), "body_steps_per_s": n / (ms * 1e-3),
-                         "vs_reference_path": round(ref_ms / ms, 3),
+                         "vs_reference_path": round(ref_ms / ms, 3) if ref_ms else None,
                          "ms_per_step_tree_rebuilt_every_%d" % REBUILD_EVERY: round(ms_refit, 4)})
             if j == 0 and energy_steps > 0:
                 e.upload(p)
@@ -298,6 +298,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and serving samples")
     ap.add_argument("--ensemble-systems", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--only", choices=["physics", "ensemble", "serving"],
+                    help="run just this sample (for profiling) and print its JSON")
     ap.add_argument("--serve-config", action="store_true",
                     help="serving sample with the configuration's distributions (N up to 2^20, 50-500 steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -321,6 +323,16 @@ def main() -> int:
         return 2
 
     from next_amd.nbx import Engine
+
+    if args.only:
+        if args.only == "physics":
+            res = physics(args.n, None, energy_steps=0)
+        elif args.only == "ensemble":
+            res = ensemble(args.ensemble_systems, steps=8, warmup=1)
+        else:
+            res = [serving(args.serve_requests, 50.0)]
+        print(json.dumps(res))
+        return 0
 
     p = ic.uniform_sphere(args.n, seed=2)
     eng = Engine(theta=args.theta)
