@@ -116,3 +116,22 @@ def test_autotune_restores_state_and_applies_choice():
         e.upload_batch(sy)
         e.run(1e-3, 3)
         assert np.array_equal(e.download(), after)
+
+
+@pytest.mark.parametrize("mode", ["reference", "physics"])
+def test_reused_context_matches_fresh(mode):
+    """A context whose buffers were sized by earlier, smaller and larger
+    systems gives the same trajectory as a fresh one: the node layout that
+    runs ahead of the node-count read-back into the last build's buffers is
+    kept only when the tree fits them, and redone otherwise."""
+    small = ic.plummer(2000, seed=11)
+    big = ic.plummer(40000, seed=12, type_fraction_dm=0.1)
+    with make(mode, True) as e:
+        for p in (small, big, small, big):
+            e.upload(p)
+            e.run(1e-3, 3)
+            got = e.download()
+            with make(mode, True) as f:
+                f.upload(p)
+                f.run(1e-3, 3)
+                assert np.array_equal(got, f.download())
