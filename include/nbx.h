@@ -186,7 +186,9 @@ nbx_status nbx_direct(nbx_ctx* ctx, double* ax, double* ay, double* az, double* 
 
 /* Energy and momentum of the current state, FP64 with a fixed reduction
  * order: out5 = {kinetic, potential, px, py, pz}.  source 0: potential from
- * the tree walk (NBX_MODE_PHYSICS), 1: from the FP64 direct sum. */
+ * the tree walk (NBX_MODE_PHYSICS), 1: from the FP64 direct sum.  Like
+ * nbx_potential, it keeps the forces cached for the next step: a run with
+ * these calls between steps is bit for bit the run without them. */
 nbx_status nbx_energy(nbx_ctx* ctx, int source, double* out5);
 /* The same per system: out receives 5 values per system. */
 nbx_status nbx_energy_systems(nbx_ctx* ctx, int source, double* out);
@@ -222,7 +224,8 @@ nbx_status nbx_debug_tree_size(nbx_ctx* ctx, int64_t* nodes);
  * centre and half-width (geo4), mass, centre of mass (com3), quadrupole
  * (q6: xx yy zz xy xz yz), and perm = bodies in leaf order.  The moments
  * are NEXT's (octree.h:92-127) in both modes; physics mode lays these
- * records out on demand from its own build of the same tree. */
+ * records out on demand from its own build of the same tree (a tree
+ * refitted since its last build is rebuilt first). */
 nbx_status nbx_debug_tree(nbx_ctx* ctx, int32_t* depth, int32_t* leaf, int32_t* body,
                           int32_t* cmask, double* geo4, double* mass, double* com3, double* q6,
                           int32_t* perm);

@@ -61,6 +61,7 @@ struct nbx_ctx {
   DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
   DBuf scratch, dts, lvl_part;
   DBuf pnodes, paux, pot, epart, eout;     // physics-mode tree, moments, potential, energy
+  DBuf dacc;                  // forces of the diagnostic passes (potential, direct energy)
   // systems
   std::vector<int64_t> sys_n;              // bodies per system
   DBuf s_root, s_dt, s_v2, s_soff, s_sroot, s_send;
@@ -608,6 +609,17 @@ nbx_status step_device(nbx_ctx* c) {
   return finish_timing(c, true);
 }
 
+// A tree built from the current positions, for the inspection hooks.  A
+// refitted physics tree still has the cells of its last build, so it is
+// rebuilt here.
+nbx_status built_tree(nbx_ctx* c) {
+  if (c->tree_valid && !(c->physics && c->since_build > 0)) return NBX_OK;
+  NBXCHK(build_tree(c));
+  c->acc_valid = false;   // the build reordered the bodies
+  c->since_build = 0;
+  return NBX_OK;
+}
+
 nbx_status check_ctx(nbx_ctx* c) {
   if (!c) return NBX_ERR_ARG;
   HIPCHK(c, hipSetDevice(c->cfg.device));
@@ -742,8 +754,9 @@ void nbx_destroy(nbx_ctx* c) {
   for (auto& b : c->amag) release(b);
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
-                  &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->epart, &c->eout, &c->s_root, &c->s_dt, &c->s_v2,
-                  &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part, &c->tgroups, &c->gmarks})
+                  &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->dacc, &c->epart, &c->eout, &c->s_root, &c->s_dt,
+                  &c->s_v2, &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part,
+                  &c->tgroups, &c->gmarks})
     release(*b);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->ev_scalars) (void)hipEventDestroy(c->ev_scalars);
@@ -1024,19 +1037,24 @@ static nbx_status download_unpermuted(nbx_ctx* c, const void* dev, double* out) 
   return download_range(c, dev, 8, 0, c->n, 0, id, out);
 }
 
-// Tree potential of the current positions (physics mode).
+// Tree potential of the current positions (physics mode).  Its walk writes
+// forces into the diagnostic buffer: the cached accelerations the next
+// step's opening kick uses stay untouched (with the relative criterion they
+// came from a walk that depended on the previous |a|), so a diagnostic call
+// between steps leaves the trajectory bit for bit as it was.
 static nbx_status tree_potential(nbx_ctx* c) {
-  NBXCHK(ensure(c, c->pot, (size_t)std::max<int64_t>(c->n, 1) * 8));
+  const size_t n = (size_t)std::max<int64_t>(c->n, 1);
+  NBXCHK(ensure(c, c->pot, n * 8));
+  NBXCHK(ensure(c, c->dacc, n * 3 * 8));
   if (!c->tree_valid) {
     NBXCHK(build_tree(c));
-    c->acc_valid = false;
+    c->acc_valid = false;   // the build reordered the bodies
   }
+  double* da = P<double>(c->dacc);
   launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
-                   c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]),
-                   P<double>(c->pot), c->sc, false, false, c->cfg.precision == NBX_PREC_FP64, nullptr, 0.0,
-                   c->cfg.theta, c->stream);
+                   c->cfg.leaf_size, da, da + n, da + 2 * n, P<double>(c->pot), c->sc, false, false,
+                   c->cfg.precision == NBX_PREC_FP64, nullptr, 0.0, c->cfg.theta, c->stream);
   HIPCHK(c, hipGetLastError());
-  c->acc_valid = true;
   return NBX_OK;
 }
 
@@ -1077,9 +1095,10 @@ nbx_status nbx_energy_systems(nbx_ctx* c, int source, double* out) {
   if (source == 0) {
     NBXCHK(tree_potential(c));
   } else {
-    launch_direct64(c->n, view(c, c->cur), sysa(c), c->cfg.eps, P<double>(c->acc[0]), P<double>(c->acc[1]),
-                    P<double>(c->acc[2]), P<double>(c->pot), c->stream);
-    c->acc_valid = false;   // the direct-sum forces replaced the tree forces
+    NBXCHK(ensure(c, c->dacc, (size_t)c->n * 3 * 8));
+    double* da = P<double>(c->dacc);
+    launch_direct64(c->n, view(c, c->cur), sysa(c), c->cfg.eps, da, da + c->n, da + 2 * c->n, P<double>(c->pot),
+                    c->stream);
   }
   launch_energy(view(c, c->cur), P<double>(c->pot), P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), S,
                 P<double>(c->epart), P<double>(c->eout), c->stream);
@@ -1147,10 +1166,7 @@ nbx_status nbx_debug_tree_size(nbx_ctx* c, int64_t* nodes) {
   if (!nodes) return fail(c, NBX_ERR_ARG, "null output");
   if (c->n == 0) { *nodes = 0; return NBX_OK; }
   if (c->sys_n.size() != 1) return fail(c, NBX_ERR_ARG, "parity hooks need a single system");
-  if (!c->tree_valid) {
-    NBXCHK(build_tree(c));
-    c->acc_valid = false;
-  }
+  NBXCHK(built_tree(c));
   *nodes = c->total_nodes;
   return NBX_OK;
 }
@@ -1161,10 +1177,7 @@ nbx_status nbx_debug_groups(nbx_ctx* c, int64_t cap, int32_t* first, int32_t* co
   if (!ngroups || cap < 0 || (cap > 0 && !(first && count && sys))) return fail(c, NBX_ERR_ARG, "bad output");
   *ngroups = 0;
   if (c->n == 0) return NBX_OK;
-  if (!c->tree_valid) {
-    NBXCHK(build_tree(c));
-    c->acc_valid = false;
-  }
+  NBXCHK(built_tree(c));
   const int64_t ng = c->physics ? c->ntgroups : c->ngroups;
   const Group* dg = P<Group>(c->physics ? c->tgroups : c->groups);
   *ngroups = ng;

@@ -313,7 +313,6 @@ class Engine:
         out["perm"] = t["perm"][: self.n]
         return out
 
-
     def autotune(self, candidates, dt: float, steps: int = 2):
         """Time each (leaf_size, group_size) candidate on the current systems
         (physics mode), keep the fastest; returns (best, ms per step of each)."""

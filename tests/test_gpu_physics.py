@@ -177,6 +177,48 @@ def test_tree_refit_between_rebuilds():
             e.set_rebuild_interval(2)
 
 
+@pytest.mark.parametrize("alpha,every", [(0.0, 1), (1e-3, 3)])
+def test_diagnostics_leave_trajectory_unchanged(alpha, every):
+    """Energies (tree and direct) and the tree potential computed between
+    steps keep the forces cached for the next step: the run with them ends
+    bit for bit where the run without them does, also with the relative
+    criterion (forces that depend on the previous step's |a|) and refits."""
+    p = ic.plummer(5000, seed=41, virial=None)
+    out = []
+    for diag in (False, True):
+        with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False,
+                    rebuild_every=every) as e:
+            e.set_alpha(alpha)
+            e.upload(p)
+            for _ in range(5):
+                e.run(1e-3, 1)
+                if diag:
+                    e.energy(direct=False)
+                    e.energy(direct=True)
+                    e.potential()
+            out.append(np.stack(e.download()))
+    assert np.array_equal(out[0], out[1])
+
+
+def test_debug_tree_after_a_refit():
+    """After a step that refitted the tree, the inspection hooks describe the
+    tree of the current positions: the same table as a fresh context's."""
+    p = ic.plummer(4000, seed=43, virial=None)
+    with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False,
+                rebuild_every=4) as e:
+        e.upload(p)
+        e.run(1e-2, 2)                                # the second step refits
+        got = e.debug_tree()
+        x, y, z, vx, vy, vz = e.download()
+    q = ic.Particles.from_columns(np.stack([x, y, z, vx, vy, vz, p.m], 1), p.type)
+    with phys() as f:
+        f.upload(q)
+        want = f.debug_tree()
+    assert got.keys() == want.keys()
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
 def subtree_ranges(depth, leaf):
     """Body range [lo, hi) in leaf order of every node of a preorder table."""
     n_nodes = len(depth)
