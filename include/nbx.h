@@ -106,6 +106,14 @@ void nbx_destroy(nbx_ctx* ctx);
 const char* nbx_last_error(const nbx_ctx* ctx);
 nbx_status nbx_set_theta(nbx_ctx* ctx, double theta);
 
+/* Device memory (no counterpart in NEXT).  Buffers grow with the bodies a
+ * context holds and are kept for reuse; nbx_trim frees them once the context
+ * holds no bodies (NBX_ERR_STATE otherwise), and nbx_device_bytes reports
+ * what the context holds now.  The request server trims a batch that
+ * empties, so idle batches hold no memory. */
+nbx_status nbx_trim(nbx_ctx* ctx);
+uint64_t nbx_device_bytes(const nbx_ctx* ctx);
+
 /* Physics mode: opening criterion relative to each group's previous
  * acceleration (no counterpart in NEXT, whose criterion is geometric).  A
  * cell that passes the geometric test (distance to the group's box beyond
@@ -269,18 +277,23 @@ typedef struct nbx_request_timing {
 
 typedef struct nbx_server_info {
   int64_t queued, running, batches;
-  uint64_t bytes_in_use;
-  int64_t body_steps;   /* bodies x steps completed */
-  int64_t batch_steps;  /* batched steps launched */
+  uint64_t bytes_in_use;  /* admission estimate of the admitted requests */
+  int64_t body_steps;     /* bodies x steps completed */
+  int64_t batch_steps;    /* batched steps launched */
+  uint64_t device_bytes;  /* device memory the batch contexts hold, as of their last step */
 } nbx_server_info;
 
 nbx_status nbx_server_create(nbx_server** out, int device, uint64_t budget_bytes);
+/* Stops the server: requests not yet done are dropped.  Call it once no
+ * other thread is inside nbx_server_wait. */
 void nbx_server_destroy(nbx_server* srv);
 nbx_status nbx_server_submit(nbx_server* srv, const nbx_request* req, int64_t* id);
 /* 0 queued, 1 running, 2 done, 3 failed; negative: unknown id. */
 int nbx_server_poll(nbx_server* srv, int64_t id);
 /* Block until the request is done, copy its final positions and velocities
- * (any pointer may be null) and release it. */
+ * (any pointer may be null) and release it; returns the request's status.
+ * Of several threads waiting on one id, one receives the result and the
+ * others NBX_ERR_ARG (the id is released). */
 nbx_status nbx_server_wait(nbx_server* srv, int64_t id, double* x, double* y, double* z, double* vx, double* vy,
                            double* vz, nbx_request_timing* timing);
 nbx_status nbx_server_stats(nbx_server* srv, nbx_server_info* out);

@@ -153,6 +153,20 @@ void release(DBuf& b) {
   b.bytes = 0;
 }
 
+// Every device buffer of a context.
+template <class F>
+void each_buf(nbx_ctx* c, F f) {
+  for (auto& w : c->st) for (auto& b : w) f(b);
+  for (DBuf* a : {c->acc, c->key, c->idx, c->key2, c->amag})
+    for (int k = 0; k < (a == c->acc ? 3 : 2); ++k) f(a[k]);
+  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
+                  &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
+                  &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->dacc, &c->epart, &c->eout, &c->s_root, &c->s_dt,
+                  &c->s_v2, &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part,
+                  &c->tgroups, &c->gmarks})
+    f(*b);
+}
+
 size_t elem_bytes(int k) { return k < 7 ? 8 : 4; }
 
 uint32_t cap_words(const DBuf& b, size_t esz) { return (uint32_t)std::min<size_t>(b.bytes / esz, UINT32_MAX); }
@@ -746,18 +760,7 @@ void nbx_destroy(nbx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& w : c->st) for (auto& b : w) release(b);
-  for (auto& b : c->acc) release(b);
-  for (auto& b : c->key) release(b);
-  for (auto& b : c->idx) release(b);
-  for (auto& b : c->key2) release(b);
-  for (auto& b : c->amag) release(b);
-  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
-                  &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
-                  &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->dacc, &c->epart, &c->eout, &c->s_root, &c->s_dt,
-                  &c->s_v2, &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part,
-                  &c->tgroups, &c->gmarks})
-    release(*b);
+  each_buf(c, release);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->ev_scalars) (void)hipEventDestroy(c->ev_scalars);
   if (c->sc) (void)hipFree(c->sc);
@@ -767,6 +770,25 @@ void nbx_destroy(nbx_ctx* c) {
 }
 
 const char* nbx_last_error(const nbx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+nbx_status nbx_trim(nbx_ctx* c) {
+  NBXCHK(check_ctx(c));
+  if (c->n != 0) return fail(c, NBX_ERR_STATE, "only a context without bodies can release its buffers");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  each_buf(c, release);
+  c->cap = 0;
+  c->skey = c->skey2 = nullptr;
+  c->spec_emit = false;
+  c->total_nodes = 0;
+  return set_systems(c);   // the (empty) per-system tables, and every valid flag cleared
+}
+
+uint64_t nbx_device_bytes(const nbx_ctx* c) {
+  if (!c) return 0;
+  uint64_t total = sizeof(DevScalars);
+  each_buf(const_cast<nbx_ctx*>(c), [&](DBuf& b) { total += b.bytes; });
+  return total;
+}
 
 nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, int32_t ncand, double dt,
                         int32_t steps, double* ms_out) {

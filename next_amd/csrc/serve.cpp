@@ -9,7 +9,8 @@
 // batch as soon as it has made its steps.  Batches with different settings
 // run on their own streams, so their work overlaps on the device.  A
 // system's trajectory does not depend on the other systems of its batch, so
-// a request's result is the same alone and in any mix.
+// a request's result is the same alone and in any mix.  A batch that empties
+// frees its device buffers, so only batches with work hold memory.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -68,6 +69,7 @@ struct Batch {
   std::vector<Request*> members;    // index = system index in ctx
   std::deque<Request*> incoming;
   std::thread thread;
+  std::atomic<uint64_t> held{0};    // device bytes of ctx after its last step
 };
 
 }  // namespace
@@ -99,6 +101,20 @@ void finish(nbx_server* s, Request* r, nbx_status st, const std::string& err) {
 }
 
 // One batch: admit its newcomers, step everyone once, retire the finished.
+// Free the context's buffers if no request is running or waiting to join,
+// and publish what the context holds.
+void settle(nbx_server* s, Batch* b) {
+  if (b->members.empty()) {
+    bool idle;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      idle = b->incoming.empty();
+    }
+    if (idle) (void)nbx_trim(b->ctx);   // on failure the buffers stay and are reused
+  }
+  b->held = nbx_device_bytes(b->ctx);
+}
+
 void batch_loop(nbx_server* s, Batch* b) {
   for (;;) {
     std::vector<Request*> joining;
@@ -135,6 +151,7 @@ void batch_loop(nbx_server* s, Batch* b) {
       (void)nbx_upload_batch(b->ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                              nullptr);
       s->cv.notify_all();
+      settle(s, b);
       continue;
     }
     s->body_steps += bodies;
@@ -151,6 +168,7 @@ void batch_loop(nbx_server* s, Batch* b) {
       b->members.erase(b->members.begin() + k);
       s->cv.notify_all();
     }
+    settle(s, b);
   }
 }
 
@@ -272,22 +290,29 @@ nbx_status nbx_server_wait(nbx_server* s, int64_t id, double* x, double* y, doub
                            double* vz, nbx_request_timing* timing) {
   if (!s) return NBX_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->requests.find(id);
+  // Look the id up again on every wake-up: another thread waiting on the
+  // same id may have received the result and released the request.
+  auto it = s->requests.end();
+  s->cv.wait(lk, [&] {
+    it = s->requests.find(id);
+    return it == s->requests.end() || it->second->state >= 2;
+  });
   if (it == s->requests.end()) return NBX_ERR_ARG;
   Request* r = it->second.get();
-  s->cv.wait(lk, [&] { return r->state >= 2; });
   if (timing) {
     timing->queue_ms = r->state == 2 && r->spec.nsteps > 0 ? ms_since(r->t_submit, r->t_start) : 0.0;
     timing->latency_ms = ms_since(r->t_submit, r->t_done);
   }
-  if (r->state == 3) return r->status;
-  const size_t n = (size_t)r->spec.n;
-  double* outs[6] = {x, y, z, vx, vy, vz};
-  const std::vector<double>* src[6] = {&r->x, &r->y, &r->z, &r->vx, &r->vy, &r->vz};
-  for (int k = 0; k < 6; ++k)
-    if (outs[k]) std::memcpy(outs[k], src[k]->data(), n * sizeof(double));
-  s->requests.erase(it);   // the result has been handed over
-  return NBX_OK;
+  const nbx_status st = r->status;
+  if (r->state == 2) {
+    const size_t n = (size_t)r->spec.n;
+    double* outs[6] = {x, y, z, vx, vy, vz};
+    const std::vector<double>* src[6] = {&r->x, &r->y, &r->z, &r->vx, &r->vy, &r->vz};
+    for (int k = 0; k < 6; ++k)
+      if (outs[k]) std::memcpy(outs[k], src[k]->data(), n * sizeof(double));
+  }
+  s->requests.erase(it);   // the result (or the failure) has been handed over
+  return st;
 }
 
 nbx_status nbx_server_stats(nbx_server* s, nbx_server_info* out) {
@@ -301,6 +326,9 @@ nbx_status nbx_server_stats(nbx_server* s, nbx_server_info* out) {
   out->bytes_in_use = (uint64_t)s->in_use;
   out->body_steps = s->body_steps.load();
   out->batch_steps = s->batch_steps.load();
+  uint64_t held = 0;
+  for (auto& kv : s->batches) held += kv.second->held.load();
+  out->device_bytes = held;
   return NBX_OK;
 }
 

@@ -23,7 +23,7 @@ MODE_REFERENCE, MODE_PHYSICS = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_set_alpha",
-           "nbx_set_rebuild_interval",
+           "nbx_set_rebuild_interval", "nbx_trim", "nbx_device_bytes",
            "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
            "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
@@ -64,7 +64,7 @@ class RequestTiming(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("queued", C.c_int64), ("running", C.c_int64), ("batches", C.c_int64), ("bytes_in_use", C.c_uint64),
-                ("body_steps", C.c_int64), ("batch_steps", C.c_int64)]
+                ("body_steps", C.c_int64), ("batch_steps", C.c_int64), ("device_bytes", C.c_uint64)]
 
 
 _lib = None
@@ -92,6 +92,9 @@ def lib():
     L.nbx_set_theta.argtypes = [VP, C.c_double]
     L.nbx_set_alpha.argtypes = [VP, C.c_double]
     L.nbx_set_rebuild_interval.argtypes = [VP, C.c_int32]
+    L.nbx_trim.argtypes = [VP]
+    L.nbx_device_bytes.argtypes = [VP]
+    L.nbx_device_bytes.restype = C.c_uint64
     L.nbx_upload.argtypes = [VP, C.c_int64] + [f64p] * 7 + [i32p]
     L.nbx_download.argtypes = [VP] + [f64p] * 6
     L.nbx_size.argtypes = [VP]
@@ -187,6 +190,14 @@ class Engine:
     def set_rebuild_interval(self, k: int):
         """Physics mode: full tree build every k-th step, refits between."""
         self._check(lib().nbx_set_rebuild_interval(self._h, k))
+
+    def trim(self):
+        """Free the device buffers of a context that holds no bodies."""
+        self._check(lib().nbx_trim(self._h))
+
+    def device_bytes(self) -> int:
+        """Device memory the context holds."""
+        return int(lib().nbx_device_bytes(self._h))
 
     def upload(self, p: Particles):
         n = len(p)

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from next_amd import ic
-from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
+from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine, NbxError
 
 pytestmark = pytest.mark.gpu
 
@@ -135,3 +135,26 @@ def test_reused_context_matches_fresh(mode):
                 f.upload(p)
                 f.run(1e-3, 3)
                 assert np.array_equal(got, f.download())
+
+
+@pytest.mark.parametrize("mode", ["reference", "physics"])
+def test_trim_frees_buffers_and_context_stays_usable(mode):
+    """nbx_trim frees an empty context's device buffers (and refuses one that
+    holds bodies); the trimmed context then runs like a fresh one."""
+    p = ic.plummer(30000, seed=13)
+    with make(mode, True) as e:
+        e.upload(p)
+        e.run(1e-3, 2)
+        held = e.device_bytes()
+        with pytest.raises(NbxError):
+            e.trim()
+        e.upload(ic.Particles.from_columns(np.zeros((0, 7)), np.zeros(0, np.int32)))
+        e.trim()
+        assert e.device_bytes() < held // 100
+        e.upload(p)
+        e.run(1e-3, 3)
+        got = e.download()
+    with make(mode, True) as f:
+        f.upload(p)
+        f.run(1e-3, 3)
+        assert np.array_equal(got, f.download())

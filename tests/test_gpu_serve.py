@@ -1,5 +1,6 @@
 """The continuous-batching request loop: each request's result equals the
 same simulation run alone; admission respects the memory budget."""
+import ctypes as C
 import threading
 import time
 
@@ -7,7 +8,8 @@ import numpy as np
 import pytest
 
 from next_amd import ic
-from next_amd.nbx import MODE_PHYSICS, MODE_REFERENCE, PREC_FP64, PREC_MIXED, Engine, NbxError, Server
+from next_amd.nbx import (MODE_PHYSICS, MODE_REFERENCE, PREC_FP64, PREC_MIXED, Engine, NbxError, RequestTiming, Server,
+                          lib)
 
 pytestmark = pytest.mark.gpu
 
@@ -71,3 +73,32 @@ def test_budget_is_respected():
     assert peak[0] <= budget
     for r, (state, _, _) in zip(rs, res):
         assert np.array_equal(state, alone(dict(r, mode=MODE_PHYSICS, precision=PREC_MIXED, theta=0.5)))
+
+
+def test_idle_batches_release_memory_and_waits_are_safe():
+    """A batch that empties frees its device buffers (the server's
+    device_bytes falls to the few bytes an empty context keeps), and two
+    threads waiting on one request get the result once and NBX_ERR_ARG once."""
+    p = ic.plummer(20000, seed=77)
+    with Server(budget_bytes=200 << 30) as srv:
+        rid = srv.submit(p, 3, 1e-3)
+        n = len(p)
+        got = []
+
+        def waiter():
+            out = [np.zeros(n) for _ in range(6)]
+            tm = RequestTiming()
+            got.append(lib().nbx_server_wait(srv._h, rid, *out, C.byref(tm)))
+
+        ts = [threading.Thread(target=waiter) for _ in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        assert sorted(got) == [-1, 0]                # NBX_ERR_ARG, NBX_OK
+        deadline = time.time() + 10
+        while srv.stats()["device_bytes"] > 1 << 20 and time.time() < deadline:
+            time.sleep(0.01)
+        st = srv.stats()
+        assert st["batches"] == 1 and st["running"] == 0 and st["bytes_in_use"] == 0
+        assert st["device_bytes"] <= 1 << 20         # was ~20,000 bodies' worth (> 10 MB)
