@@ -318,7 +318,7 @@ def main() -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this engine runs one particle system per GPU and is measured on one GPU only "
-              f"(--gpus {args.gpus}, WORLD_SIZE {world} requested); see DESIGN.md for how it would shard.",
+              f"(--gpus {args.gpus}, WORLD_SIZE {world} requested); multi-GPU sharding is not built.",
               file=sys.stderr)
         return 2
 

@@ -50,7 +50,7 @@ def test_null_and_bad_arguments_are_rejected_without_a_device():
 
 def test_create_reports_the_device_state():
     """On a machine without a GPU nbx_create fails with NBX_ERR_NODEVICE; on
-    a GPU box it succeeds."""
+    a machine with one it succeeds."""
     try:
         e = nbx.Engine()
     except nbx.NbxError as err:
@@ -69,7 +69,7 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 def test_next_interface_mirror_compiles_and_runs(tmp_path):
     """include/nbx_next.hpp offers NEXT's computeAdaptiveDt/Step signatures
     for a struct with NEXT's Particle fields; built and run here it reports
-    the missing GPU as nbx_next::Error (NBX_ERR_NODEVICE), on a GPU box it
+    the missing GPU as nbx_next::Error (NBX_ERR_NODEVICE); with a GPU it
     steps."""
     import subprocess
     src = tmp_path / "use.cpp"
