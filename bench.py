@@ -27,6 +27,7 @@ import os
 import subprocess
 import sys
 import tempfile
+import threading
 import time
 
 import numpy as np
@@ -233,7 +234,19 @@ def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(2
         if (k + 1) % 50 == 0:
             log("serving at %g/s: %d of %d requests generated" % (rate, k + 1, nreq))
     lat, ids = [], []
+    peak = {"device_bytes": 0, "bytes_in_use": 0}
     with Server(budget_bytes=200 << 30) as srv:
+        done = threading.Event()
+
+        def watch():                     # device memory held by the batches, and the admission estimate
+            while not done.is_set():
+                cur = srv.stats()
+                for key in peak:
+                    peak[key] = max(peak[key], cur[key])
+                time.sleep(0.005)
+
+        watcher = threading.Thread(target=watch)
+        watcher.start()
         t0 = time.perf_counter()
         for r in reqs:
             time.sleep(r["gap"])
@@ -245,10 +258,13 @@ def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(2
             if (k + 1) % 25 == 0:
                 log("serving at %g/s: %d of %d requests done" % (rate, k + 1, nreq))
         wall = time.perf_counter() - t0
+        done.set()
+        watcher.join()
         st = srv.stats()
     return {"requests": nreq, "arrival_rate_per_s": rate, "p50_latency_ms": float(np.percentile(lat, 50)),
             "p99_latency_ms": float(np.percentile(lat, 99)), "body_steps_per_s": st["body_steps"] / wall,
-            "batches": st["batches"],
+            "batches": st["batches"], "peak_device_gb": round(peak["device_bytes"] / 1e9, 3),
+            "peak_admitted_estimate_gb": round(peak["bytes_in_use"] / 1e9, 3),
             "workload": f"Poisson arrivals; Plummer (DF sample) or cold sphere, N log-uniform [4096, {n_max}], "
                         f"{steps[0]}-{steps[1]} steps, theta {{0.4, 0.5, 0.7}}, FP32/FP64 walk, dt 1e-3, eps 0.01, "
                         "200 GB budget"}

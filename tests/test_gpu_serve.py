@@ -102,3 +102,23 @@ def test_idle_batches_release_memory_and_waits_are_safe():
         st = srv.stats()
         assert st["batches"] == 1 and st["running"] == 0 and st["bytes_in_use"] == 0
         assert st["device_bytes"] <= 1 << 20         # was ~20,000 bodies' worth (> 10 MB)
+
+
+@pytest.mark.parametrize("mode", [MODE_REFERENCE, MODE_PHYSICS])
+def test_admission_estimate_covers_a_context(mode):
+    """The server admits requests by an estimate of BYTES_PER_BODY device
+    bytes per body; a context that has stepped a system and computed its
+    diagnostics (every buffer a request can make it allocate) holds less."""
+    p = ic.plummer(100000, seed=78)
+    prec = PREC_MIXED if mode == MODE_PHYSICS else PREC_FP64
+    with Engine(theta=0.5, mode=mode, precision=prec, eps=0.01, leaf_size=16, adaptive_dt=True) as e:
+        if mode == MODE_PHYSICS:
+            e.set_alpha(1e-3)
+        e.upload(p)
+        e.run(1e-3, 3)
+        e.energy(direct=True)
+        if mode == MODE_PHYSICS:
+            e.energy(direct=False)
+        per_body = e.device_bytes() / len(p)
+    print(f"device bytes per body: {per_body:.0f}")
+    assert per_body <= BYTES_PER_BODY
