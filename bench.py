@@ -13,7 +13,8 @@ VALU; algorithmic flops 19 per particle-particle and 55 per particle-cell
 interaction, counted on the device; HBM traffic from the committed PMC
 summary), the per-phase split with the HBM rate of the streaming phases, the
 mixed-precision physics mode on the same workload (step time, force error
-against the FP64 direct sum, energy drift), a bounded sample of the
+against the FP64 direct sum, energy drift, the FP32 walk's FLOP/s against
+the FP32 peak), a bounded sample of the
 batched-ensemble and serving workloads (physics mode), and NEXT's own CPU
 code timed on this host on a bounded sample of the headline workload.
 
@@ -38,8 +39,14 @@ sys.path.insert(0, ROOT)
 from next_amd import ic  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed FP32; unpacked issue gives half)
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP, FLOPS_PC = 19, 55
+# Physics-mode FP32 walk, flops as written in k_phys_walk (rsqrt counted as
+# one): particle-particle 3 (d) + 6 (r^2 + eps^2) + 1 + 3 (m r^-3) + 6 (f) = 19;
+# particle-cell 3 + 6 + 1 + 3 (r^-2, r^-3, r^-5) + 1 (Qzz) + 15 (Q d) + 5 (d.Q.d)
+# + 1 (m r^-3) + 3 (2.5 d.Q.d r^-7) + 1 + 12 (f) = 51.
+PHYS_FLOPS_PP, PHYS_FLOPS_PC = 19, 51
 REBUILD_EVERY = 4         # ensembles: full tree build every 4th step, refits between
 
 
@@ -198,10 +205,27 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
             e.run(5e-4, 3 * REBUILD_EVERY)
             ms_refit = (time.perf_counter() - t0) * 1e3 / (3 * REBUILD_EVERY)
             e.set_rebuild_interval(1)
+            # the walk kernel alone: HIP-event time, then device-counted interactions
+            e.set_profiling(1)
+            e.reset_timing()
+            e.run(5e-4, 3)
+            tim = e.timing()
+            walk_ms = tim["walk_ms"] / tim["steps"]
+            e.set_profiling(2)
+            e.reset_timing()
+            e.run(5e-4, 1)
+            cnt = e.timing()
+            e.set_profiling(0)
+            tflops = (PHYS_FLOPS_PP * cnt["n_pp"] + PHYS_FLOPS_PC * cnt["n_pc"]) / (walk_ms * 1e-3) / 1e12
             rows.append({"theta": theta, "ms_per_step": round(ms, 4), "median_rel_force_err": float(np.median(err)),
                          "p99_rel_force_err": float(np.percentile(err, 99)), "body_steps_per_s": n / (ms * 1e-3),
                          "vs_reference_path": round(ref_ms / ms, 3) if ref_ms else None,
-                         "ms_per_step_tree_rebuilt_every_%d" % REBUILD_EVERY: round(ms_refit, 4)})
+                         "ms_per_step_tree_rebuilt_every_%d" % REBUILD_EVERY: round(ms_refit, 4),
+                         "walk": {"ms": round(walk_ms, 4), "pp_per_body": round(cnt["n_pp"] / n, 1),
+                                  "pc_per_body": round(cnt["n_pc"] / n, 1), "achieved": round(tflops, 2),
+                                  "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                                  "flops_per_pp_pc": [PHYS_FLOPS_PP, PHYS_FLOPS_PC]}})
             if j == 0 and energy_steps > 0:
                 e.upload(p)
                 e0 = e.energy(direct=True)
