@@ -173,14 +173,16 @@ def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, ener
             "vs_plain_fp64_path": round(seq * 1e3 / ms, 2)}
 
 
-def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) -> dict:
+def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, fp64_walk: bool = False) -> dict:
     """Mixed-precision physics mode (FP32 walk, FP64 state) on the headline
     workload: cold uniform sphere of n bodies, eps 0.005, fixed dt 5e-4.  Per
     theta: ms per KDK step, force error against the FP64 direct sum on 20,000
     sampled bodies, and per-body throughput relative to the reference-mode
     step (ref_ms); |dE/E| (FP64 direct-sum energies) over energy_steps steps
-    at the first theta."""
+    at the first theta.  fp64_walk: the same with the FP64 walk."""
     from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Engine
+
+    walk_prec = PREC_FP64 if fp64_walk else PREC_MIXED
 
     p = ic.uniform_sphere(n, seed=2)
     with Engine(mode=MODE_PHYSICS, theta=0.5, eps=0.005, precision=PREC_FP64) as e:
@@ -189,7 +191,7 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
     sub = np.random.Generator(np.random.PCG64(0)).choice(n, min(n, 20000), replace=False)
     rows, energy = [], None
     for j, theta in enumerate(thetas):
-        with Engine(mode=MODE_PHYSICS, theta=theta, eps=0.005, precision=PREC_MIXED, leaf_size=16, group_size=64,
+        with Engine(mode=MODE_PHYSICS, theta=theta, eps=0.005, precision=walk_prec, leaf_size=16, group_size=64,
                     adaptive_dt=False) as e:
             e.upload(p)
             a = e.accel()
@@ -217,14 +219,14 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
             cnt = e.timing()
             e.set_profiling(0)
             tflops = (PHYS_FLOPS_PP * cnt["n_pp"] + PHYS_FLOPS_PC * cnt["n_pc"]) / (walk_ms * 1e-3) / 1e12
+            peak = FP64_PEAK_TFLOPS if fp64_walk else FP32_PEAK_TFLOPS
             rows.append({"theta": theta, "ms_per_step": round(ms, 4), "median_rel_force_err": float(np.median(err)),
                          "p99_rel_force_err": float(np.percentile(err, 99)), "body_steps_per_s": n / (ms * 1e-3),
                          "vs_reference_path": round(ref_ms / ms, 3) if ref_ms else None,
                          "ms_per_step_tree_rebuilt_every_%d" % REBUILD_EVERY: round(ms_refit, 4),
                          "walk": {"ms": round(walk_ms, 4), "pp_per_body": round(cnt["n_pp"] / n, 1),
                                   "pc_per_body": round(cnt["n_pc"] / n, 1), "achieved": round(tflops, 2),
-                                  "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                  "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                                  "peak": peak, "unit": "TFLOP/s", "frac": round(tflops / peak, 4),
                                   "flops_per_pp_pc": [PHYS_FLOPS_PP, PHYS_FLOPS_PC]}})
             if j == 0 and energy_steps > 0:
                 e.upload(p)
@@ -235,7 +237,8 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200) 
                 energy = {"theta": theta, "steps": energy_steps, "E0": E0, "E1": E1,
                           "abs_dE_over_E": abs(E1 - E0) / abs(E0)}
         log("physics theta %g: %.3f ms/step, median force error %.2e" % (theta, ms, rows[-1]["median_rel_force_err"]))
-    return {"workload": f"cold uniform sphere N={n}, eps 0.005, fixed dt 5e-4, leaf 16, group 64, FP32 walk / FP64 state",
+    return {"workload": f"cold uniform sphere N={n}, eps 0.005, fixed dt 5e-4, leaf 16, group 64, "
+                        f"{'FP64' if fp64_walk else 'FP32'} walk / FP64 state",
             "by_theta": rows, "energy": energy}
 
 
@@ -340,6 +343,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--only", choices=["physics", "ensemble", "serving"],
                     help="run just this sample (for profiling) and print its JSON")
+    ap.add_argument("--thetas", default="0.32,0.5", help="physics sample: opening angles (comma-separated)")
+    ap.add_argument("--walk-precision", choices=["fp32", "fp64"], default="fp32", help="physics sample: walk precision")
     ap.add_argument("--serve-config", action="store_true",
                     help="serving sample with the configuration's distributions (N up to 2^20, 50-500 steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -366,7 +371,8 @@ def main() -> int:
 
     if args.only:
         if args.only == "physics":
-            res = physics(args.n, None, energy_steps=0)
+            res = physics(args.n, None, thetas=[float(t) for t in args.thetas.split(",")], energy_steps=0,
+                          fp64_walk=args.walk_precision == "fp64")
         elif args.only == "ensemble":
             res = ensemble(args.ensemble_systems, steps=8, warmup=1)
         else:
