@@ -242,6 +242,50 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, 
             "by_theta": rows, "energy": energy}
 
 
+def merger(n_each: int = 1 << 21, theta: float = 0.5, steps: int = 500, sample_steps: int = 20) -> dict:
+    """Two-cluster merger (two Plummer spheres of n_each bodies, a = 1, M =
+    0.5 each, at (+-5, 0, 0) with bulk velocities (-+0.3, +-0.1, 0)), G = 1,
+    eps 0.01, dt 1e-3, physics mode with FP32 forces and FP64 state: force
+    error against the FP64 direct sum on 20,000 sampled bodies, ms per step
+    over sample_steps steps, then the full run of `steps` steps (wall time and
+    |dE/E| from FP64 direct-sum energies)."""
+    from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
+
+    p = ic.two_cluster(n_each, seed=3)
+    n = len(p)
+    with Engine(mode=MODE_PHYSICS, theta=theta, eps=0.01, precision=PREC_MIXED, leaf_size=16, group_size=64,
+                adaptive_dt=False) as e:
+        e.upload(p)
+        a_ref, _ = e.direct()
+        a = e.accel()
+        sub = np.random.Generator(np.random.PCG64(0)).choice(n, 20000, replace=False)
+        d = np.linalg.norm(a_ref[sub], axis=1)
+        err = np.linalg.norm(a[sub] - a_ref[sub], axis=1) / np.where(d > 0, d, 1.0)
+        log("merger: force error median %.2e" % np.median(err))
+        e.run(1e-3, 2)
+        t0 = time.perf_counter()
+        e.run(1e-3, sample_steps)
+        ms = (time.perf_counter() - t0) * 1e3 / sample_steps
+        log("merger: %.3f ms/step" % ms)
+        e.upload(p)
+        e0 = e.energy(direct=True)
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:                      # in chunks, so progress is visible
+            k = min(100, steps - done)
+            e.run(1e-3, k)
+            done += k
+            log("merger: %d of %d steps" % (done, steps))
+        wall = time.perf_counter() - t0
+        e1 = e.energy(direct=True)
+    E0, E1 = e0[0] + e0[1], e1[0] + e1[1]
+    return {"workload": f"two Plummer clusters, 2 x {n_each} bodies, theta {theta}, eps 0.01, dt 1e-3, "
+                        "leaf 16, group 64, FP32 walk / FP64 state",
+            "median_rel_force_err": float(np.median(err)), "p99_rel_force_err": float(np.percentile(err, 99)),
+            "ms_per_step": round(ms, 3), "body_steps_per_s": n / (ms * 1e-3),
+            "full_run": {"steps": steps, "wall_s": round(wall, 2), "abs_dE_over_E": abs(E1 - E0) / abs(E0)}}
+
+
 def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(20, 60)) -> dict:
     """Serving sample: Poisson arrivals at `rate` per second of independent
     Plummer or cold-sphere runs, N log-uniform in [4096, n_max], step counts
@@ -341,7 +385,7 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and serving samples")
     ap.add_argument("--ensemble-systems", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--only", choices=["physics", "ensemble", "serving"],
+    ap.add_argument("--only", choices=["physics", "ensemble", "serving", "merger"],
                     help="run just this sample (for profiling) and print its JSON")
     ap.add_argument("--thetas", default="0.32,0.5", help="physics sample: opening angles (comma-separated)")
     ap.add_argument("--walk-precision", choices=["fp32", "fp64"], default="fp32", help="physics sample: walk precision")
@@ -375,6 +419,8 @@ def main() -> int:
                           fp64_walk=args.walk_precision == "fp64")
         elif args.only == "ensemble":
             res = ensemble(args.ensemble_systems, steps=8, warmup=1)
+        elif args.only == "merger":
+            res = merger()
         else:
             res = [serving(args.serve_requests, 50.0)]
         print(json.dumps(res))
