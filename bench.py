@@ -42,11 +42,12 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed FP32; unpacked issue gives half)
 HBM_PEAK_GBS = 8000.0
 FLOPS_PP, FLOPS_PC = 19, 55
-# Physics-mode FP32 walk, flops as written in k_phys_walk (rsqrt counted as
-# one): particle-particle 3 (d) + 6 (r^2 + eps^2) + 1 + 3 (m r^-3) + 6 (f) = 19;
-# particle-cell 3 + 6 + 1 + 3 (r^-2, r^-3, r^-5) + 1 (Qzz) + 15 (Q d) + 5 (d.Q.d)
-# + 1 (m r^-3) + 3 (2.5 d.Q.d r^-7) + 1 + 12 (f) = 51.
-PHYS_FLOPS_PP, PHYS_FLOPS_PC = 19, 51
+# The physics walk is counted by the same convention (FMA = 2, rsqrt = 1,
+# potential excluded): particle-particle 3 (d) + 6 (r^2 + eps^2) + 1 + 1 (r^-2)
+# + 2 (m r^-3) + 6 (accumulate) = 19; particle-cell with the traceless
+# quadrupole 19 + 15 (Q d) + 5 (d.Q.d) + 2 (r^-5, r^-7) + 2 (coefficient) + 12
+# = 55.  k_phys_walk as written shares some products and issues about 51.
+PHYS_FLOPS_PP, PHYS_FLOPS_PC = FLOPS_PP, FLOPS_PC
 REBUILD_EVERY = 4         # ensembles: full tree build every 4th step, refits between
 
 
@@ -225,7 +226,9 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, 
                          "vs_reference_path": round(ref_ms / ms, 3) if ref_ms else None,
                          "ms_per_step_tree_rebuilt_every_%d" % REBUILD_EVERY: round(ms_refit, 4),
                          "walk": {"ms": round(walk_ms, 4), "pp_per_body": round(cnt["n_pp"] / n, 1),
-                                  "pc_per_body": round(cnt["n_pc"] / n, 1), "achieved": round(tflops, 2),
+                                  "pc_per_body": round(cnt["n_pc"] / n, 1),
+                                  "interactions_per_s": (cnt["n_pp"] + cnt["n_pc"]) / (walk_ms * 1e-3),
+                                  "achieved": round(tflops, 2),
                                   "peak": peak, "unit": "TFLOP/s", "frac": round(tflops / peak, 4),
                                   "flops_per_pp_pc": [PHYS_FLOPS_PP, PHYS_FLOPS_PC]}})
             if j == 0 and energy_steps > 0:
