@@ -2,7 +2,8 @@
 //
 // A request is one independent simulation: its bodies, a step count and its
 // settings.  Requests are admitted first come, first served while their
-// estimated device memory fits the budget.  Admitted requests with the same
+// estimated device memory fits the budget (counting what the batch contexts
+// actually hold instead, when that is more).  Admitted requests with the same
 // settings share one batch context (one HIP stream, one host thread): a
 // request joins its batch between two steps, every step advances all of the
 // batch's systems with one set of kernel launches, and a request leaves its
@@ -11,6 +12,7 @@
 // system's trajectory does not depend on the other systems of its batch, so
 // a request's result is the same alone and in any mix.  A batch that empties
 // frees its device buffers, so only batches with work hold memory.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -112,7 +114,12 @@ void settle(nbx_server* s, Batch* b) {
     }
     if (idle) (void)nbx_trim(b->ctx);   // on failure the buffers stay and are reused
   }
-  b->held = nbx_device_bytes(b->ctx);
+  const uint64_t bytes = nbx_device_bytes(b->ctx);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);   // the scheduler admits against it
+    b->held = bytes;
+  }
+  s->cv.notify_all();
 }
 
 void batch_loop(nbx_server* s, Batch* b) {
@@ -172,15 +179,23 @@ void batch_loop(nbx_server* s, Batch* b) {
   }
 }
 
-// Admission: first come, first served, while the estimated memory fits.
+// Bytes counted against the budget (caller holds the lock): the admitted
+// requests' estimates, or what the batch contexts hold if that is more
+// (their buffers grow with slack and stay until a batch empties).
+size_t committed(const nbx_server* s) {
+  uint64_t held = 0;
+  for (const auto& kv : s->batches) held += kv.second->held.load();
+  return std::max<size_t>(s->in_use, (size_t)held);
+}
+
+// Admission: first come, first served, while the memory fits.
 void scheduler_loop(nbx_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
+  auto fits = [&] { return !s->pending.empty() && committed(s) + s->pending.front()->bytes <= s->budget; };
   for (;;) {
-    s->cv.wait(lk, [&] {
-      return s->stop || (!s->pending.empty() && s->in_use + s->pending.front()->bytes <= s->budget);
-    });
+    s->cv.wait(lk, [&] { return s->stop || fits(); });
     if (s->stop) return;
-    while (!s->pending.empty() && s->in_use + s->pending.front()->bytes <= s->budget) {
+    while (fits()) {
       Request* r = s->pending.front();
       s->pending.pop_front();
       const BatchKey key{r->spec.mode, r->spec.precision, r->spec.leaf_size, r->spec.adaptive_dt, r->spec.theta,

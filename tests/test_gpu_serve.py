@@ -55,12 +55,14 @@ def test_results_equal_alone_and_overlap():
 def test_budget_is_respected():
     rs = [dict(p=ic.plummer(3000, seed=k), nsteps=4, dt=1e-3) for k in range(5)]
     budget = int(3000 * BYTES_PER_BODY * 1.5)     # one request at a time
-    peak = [0]
+    peak = [0, 0]
     stop = threading.Event()
     with Server(budget_bytes=budget) as srv:
         def watch():
             while not stop.is_set():
-                peak[0] = max(peak[0], srv.stats()["bytes_in_use"])
+                st = srv.stats()
+                peak[0] = max(peak[0], st["bytes_in_use"])
+                peak[1] = max(peak[1], st["device_bytes"])
                 time.sleep(0.0005)
         t = threading.Thread(target=watch)
         t.start()
@@ -70,7 +72,7 @@ def test_budget_is_respected():
         t.join()
         with pytest.raises(NbxError):
             srv.submit(ic.plummer(100000, seed=1), 1, 1e-3)      # can never fit
-    assert peak[0] <= budget
+    assert peak[0] <= budget and peak[1] <= budget
     for r, (state, _, _) in zip(rs, res):
         assert np.array_equal(state, alone(dict(r, mode=MODE_PHYSICS, precision=PREC_MIXED, theta=0.5)))
 
