@@ -221,6 +221,8 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, 
             e.set_profiling(0)
             tflops = (PHYS_FLOPS_PP * cnt["n_pp"] + PHYS_FLOPS_PC * cnt["n_pc"]) / (walk_ms * 1e-3) / 1e12
             peak = FP64_PEAK_TFLOPS if fp64_walk else FP32_PEAK_TFLOPS
+            # HBM bytes per launch from the committed PMC summary of this walk (FP32, theta 0.5)
+            prof = load_profile("phys_walk_traffic.json") if theta == 0.5 and not fp64_walk else None
             rows.append({"theta": theta, "ms_per_step": round(ms, 4), "median_rel_force_err": float(np.median(err)),
                          "p99_rel_force_err": float(np.percentile(err, 99)), "body_steps_per_s": n / (ms * 1e-3),
                          "vs_reference_path": round(ref_ms / ms, 3) if ref_ms else None,
@@ -230,7 +232,8 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, 
                                   "interactions_per_s": (cnt["n_pp"] + cnt["n_pc"]) / (walk_ms * 1e-3),
                                   "achieved": round(tflops, 2),
                                   "peak": peak, "unit": "TFLOP/s", "frac": round(tflops / peak, 4),
-                                  "flops_per_pp_pc": [PHYS_FLOPS_PP, PHYS_FLOPS_PC]}})
+                                  "flops_per_pp_pc": [PHYS_FLOPS_PP, PHYS_FLOPS_PC],
+                                  "traffic": prof["bytes_per_launch"] if prof else None}})
             if j == 0 and energy_steps > 0:
                 e.upload(p)
                 e0 = e.energy(direct=True)
@@ -366,7 +369,7 @@ def phase_rooflines(n: int, tim: dict, k: int) -> dict:
 
 
 def load_profile(name: str) -> dict | None:
-    """A committed rocprofv3 summary of the walk kernel (HBM traffic or VALU
+    """A committed rocprofv3 summary of a walk kernel (HBM traffic or VALU
     issue counters), or None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
