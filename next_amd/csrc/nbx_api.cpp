@@ -69,7 +69,11 @@ struct nbx_ctx {
   int64_t ngroups = 0;
   DBuf tgroups;               // physics mode: walk groups packed from the tree
   DBuf gmarks;                // physics mode: per-body group counts and marks
+  // Physics mode: the count of walk groups lives on the device
+  // (sc->tree_groups); the host knows it after a read-back (else -1) and
+  // sizes the walk's grid from the bound meanwhile.
   int64_t ntgroups = 0;
+  int64_t groups_bound = 0;
   int nboxes = 0;
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
@@ -483,9 +487,20 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
       if (c->hsc->level_count[d] > 0)
         launch_phys_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], d, pn, c->paux.p,
                            c->cfg.theta, c->cfg.group_size, gcnt, gmark, s);
+    // The walk reads the group count on the device.  After a build of the
+    // same systems and settings its grid comes from that build's count (it
+    // reached the host with this build's scalars), with room to grow; the
+    // first build of a layout reads the count back.
+    const bool bounded = c->layout_valid;
+    const int64_t last = c->hsc->tree_groups;
     launch_group_compact(n, gcnt, gmark, St.sys, P<Group>(c->tgroups), &c->sc->tree_groups, scr, s);
-    NBXCHK(sync_scalars(c));
-    c->ntgroups = c->hsc->tree_groups;
+    if (bounded) {
+      c->ntgroups = -1;
+      c->groups_bound = last + last / 4 + 64;
+    } else {
+      NBXCHK(sync_scalars(c));
+      c->ntgroups = c->hsc->tree_groups;
+    }
     c->layout_valid = true;
   } else {
     NBXCHK(ref_layout(c, c->spec_emit && total + 1 <= (int64_t)c->spec_nodes && m <= (int64_t)c->spec_int));
@@ -538,13 +553,16 @@ nbx_status build_tree(nbx_ctx* c) {
   return build_post(c, skey);
 }
 
+// Grid of the physics walk: the group count if the host has it, else its bound.
+int64_t group_grid(const nbx_ctx* c) { return c->ntgroups >= 0 ? c->ntgroups : c->groups_bound; }
+
 nbx_status walk(nbx_ctx* c, bool kick) {
   const bool count = (c->prof & 2) != 0;
   if (count) launch_fill_scalars(c->sc, 8, c->stream);
   if (kick) launch_fill_v2max(sysa(c), c->stream);
   record(c, EV_WALK0);
   if (c->physics)
-    launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
+    launch_phys_walk(P<Group>(c->tgroups), group_grid(c), P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                      c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr,
                      c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64,
                      c->amag_live ? P<float>(c->amag[c->acur]) : nullptr, c->alpha, c->cfg.theta, c->stream);
@@ -1073,7 +1091,7 @@ static nbx_status tree_potential(nbx_ctx* c) {
     c->acc_valid = false;   // the build reordered the bodies
   }
   double* da = P<double>(c->dacc);
-  launch_phys_walk(P<Group>(c->tgroups), c->ntgroups, P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
+  launch_phys_walk(P<Group>(c->tgroups), group_grid(c), P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                    c->cfg.leaf_size, da, da + n, da + 2 * n, P<double>(c->pot), c->sc, false, false,
                    c->cfg.precision == NBX_PREC_FP64, nullptr, 0.0, c->cfg.theta, c->stream);
   HIPCHK(c, hipGetLastError());
@@ -1200,6 +1218,10 @@ nbx_status nbx_debug_groups(nbx_ctx* c, int64_t cap, int32_t* first, int32_t* co
   *ngroups = 0;
   if (c->n == 0) return NBX_OK;
   NBXCHK(built_tree(c));
+  if (c->physics && c->ntgroups < 0) {
+    NBXCHK(sync_scalars(c));
+    c->ntgroups = c->hsc->tree_groups;
+  }
   const int64_t ng = c->physics ? c->ntgroups : c->ngroups;
   const Group* dg = P<Group>(c->physics ? c->tgroups : c->groups);
   *ngroups = ng;

@@ -273,11 +273,6 @@ __device__ __forceinline__ float rsq<float>(float x) { return __frsqrt_rn(x); }
 template <>
 __device__ __forceinline__ double rsq<double>(double x) { return rsqrt(x); }
 
-__device__ __forceinline__ int64_t xcd_group(int nb) {
-  const int per = (nb + 7) / 8;
-  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-}
-
 // Group walk with work donation.  Every lane walks a contiguous range of
 // the preorder node array with skip pointers (accept or bucket: jump past the
 // subtree; open: step to the first child).  Lane 0 starts with the whole
@@ -289,7 +284,7 @@ __device__ __forceinline__ int64_t xcd_group(int nb) {
 // lane order, so the lists, and every sum, come out in the same order on
 // every run.
 template <typename T, bool KICK, bool POT, bool COUNT>
-__global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restrict__ groups, int64_t ngroups,
+__global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restrict__ groups,
                                                           const PhysNode* __restrict__ pn, StateArrays st,
                                                           SysArrays sy, double eps, int nleaf,
                                                           double* __restrict__ ax, double* __restrict__ ay,
@@ -306,233 +301,245 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   __shared__ V4<T> pl[kPhysParts];       // body: relative position, mass
 
   const int lane = threadIdx.x;
-  const int64_t gi = xcd_group(gridDim.x);
-  if (gi >= ngroups) return;                               // whole wave past the end
-  const Group g = groups[gi];
-  // A group of at most 32 (16, ...) targets puts 2 (4, ...) lanes on each
-  // target: lane = slot + part * width, and the parts evaluate interleaved
-  // entries of every list, their sums combined after each flush.
-  int width = kWave;
-  while (width > 1 && (width >> 1) >= g.count) width >>= 1;
-  const int parts = kWave / width;
-  const int slot = lane & (width - 1), part = lane / width;
-  const int64_t i = g.first + slot;
-  const bool valid = slot < g.count && part == 0;        // the lane that owns the result
-  const int64_t ii = slot < g.count ? i : g.first;
-  const double px = st.x[ii], py = st.y[ii], pz = st.z[ii];
-  const uint64_t ltmask = lanemask_lt();
-  const int32_t root = sy.sroot[g.sys], total = sy.send[g.sys];   // this system's tree
+  // The group count comes from the device (the compaction that wrote it ran
+  // earlier on this stream), so the host need not read it back first; the
+  // grid (a multiple of 8) is sized from a bound.  Blocks are dealt to the 8
+  // XCDs round-robin: XCD x walks the contiguous (spatially coherent) range
+  // [x per, (x + 1) per) of the groups with its gridDim / 8 blocks, so its L2
+  // holds the part of the tree those groups touch, whatever the grid size.
+  const int64_t ngroups = (int64_t)sc->tree_groups;
+  const int64_t per = (ngroups + 7) / 8;
+  const int64_t xbase = (int64_t)(blockIdx.x % 8) * per;
+  for (int64_t j = blockIdx.x / 8; j < per; j += gridDim.x / 8) {
+    const int64_t gi = xbase + j;
+    if (gi >= ngroups) break;                              // the last range is short
+    const Group g = groups[gi];
+    // A group of at most 32 (16, ...) targets puts 2 (4, ...) lanes on each
+    // target: lane = slot + part * width, and the parts evaluate interleaved
+    // entries of every list, their sums combined after each flush.
+    int width = kWave;
+    while (width > 1 && (width >> 1) >= g.count) width >>= 1;
+    const int parts = kWave / width;
+    const int slot = lane & (width - 1), part = lane / width;
+    const int64_t i = g.first + slot;
+    const bool valid = slot < g.count && part == 0;        // the lane that owns the result
+    const int64_t ii = slot < g.count ? i : g.first;
+    const double px = st.x[ii], py = st.y[ii], pz = st.z[ii];
+    const uint64_t ltmask = lanemask_lt();
+    const int32_t root = sy.sroot[g.sys], total = sy.send[g.sys];   // this system's tree
 
-  // group bounding box
-  double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double a = __shfl_xor(lo[k], off, kWave), b = __shfl_xor(hi[k], off, kWave);
-      lo[k] = a < lo[k] ? a : lo[k];
-      hi[k] = b > hi[k] ? b : hi[k];
-    }
-  }
-  const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]), gcz = 0.5 * (lo[2] + hi[2]);
-  const float hx = (float)(0.5 * (hi[0] - lo[0])) * (1.0f + 1e-6f);
-  const float hy = (float)(0.5 * (hi[1] - lo[1])) * (1.0f + 1e-6f);
-  const float hz = (float)(0.5 * (hi[2] - lo[2])) * (1.0f + 1e-6f);
-  const T tx = (T)(px - gcx), ty = (T)(py - gcy), tz = (T)(pz - gcz);
-  const T e2 = (T)(eps * eps);
-  // Relative criterion (amag != null): a cell that passes the geometric test
-  // is accepted only if its first neglected (octupole) term, ~ m bmax^3 / d^5,
-  // is at most alpha times the smallest previous |a| of the group:
-  // m^2 bmax^6 <= alpha^2 amin^2 d^10, with bmax^2 = rc2 * theta^2.
-  float arel = 0.f;
-  if (amag) {
-    float am = amag[ii];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) am = fminf(am, __shfl_xor(am, off, kWave));
-    arel = rel2 * am * am;
-  }
-
-  double Ax = 0, Ay = 0, Az = 0, P = 0;
-  uint32_t ncell_tot = 0, npart_tot = 0, niter = 0, nopen = 0;
-  int nc = 0, np = 0;
-
-  auto flush_cells = [&]() {
-    __syncthreads();
-    T fx = 0, fy = 0, fz = 0, fp = 0;
-    for (int k = part; k < nc; k += parts) {
-      const V4<T> a = ca[k];
-      const V4<T> b = cb[k];
-      const T qyz = cq[k];
-      const T dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
-      const T r2 = dx * dx + dy * dy + dz * dz + e2;
-      const T ri = rsq<T>(r2);
-      const T ri2 = ri * ri;
-      const T ri3 = ri2 * ri;
-      const T ri5 = ri3 * ri2;
-      const T qzz = -(b.x + b.y);
-      const T qdx = b.x * dx + b.z * dy + b.w * dz;
-      const T qdy = b.z * dx + b.y * dy + qyz * dz;
-      const T qdz = b.w * dx + qyz * dy + qzz * dz;
-      const T dqd = dx * qdx + dy * qdy + dz * qdz;
-      const T mono = a.w * ri3;
-      const T c7 = (T)2.5 * dqd * ri5 * ri2;
-      fx += (mono + c7) * dx - ri5 * qdx;
-      fy += (mono + c7) * dy - ri5 * qdy;
-      fz += (mono + c7) * dz - ri5 * qdz;
-      if (POT) fp -= a.w * ri + (T)0.5 * dqd * ri5;
-    }
-    combine_parts(fx, fy, fz, fp, width);
-    Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
-    if (POT) P += (double)fp;
-    if (COUNT) ncell_tot += nc;
-    __syncthreads();
-    nc = 0;
-  };
-  auto flush_parts = [&]() {
-    __syncthreads();
-    T fx = 0, fy = 0, fz = 0, fp = 0;
-    for (int k = part; k < np; k += parts) {
-      const V4<T> b = pl[k];
-      const T dx = b.x - tx, dy = b.y - ty, dz = b.z - tz;
-      const T r2 = dx * dx + dy * dy + dz * dz + e2;
-      const T ri = rsq<T>(r2);
-      const T mr3 = b.w * ri * ri * ri;
-      fx += mr3 * dx;
-      fy += mr3 * dy;
-      fz += mr3 * dz;
-      if (POT) fp -= b.w * ri;
-    }
-    combine_parts(fx, fy, fz, fp, width);
-    Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
-    if (POT) P += (double)fp;
-    if (COUNT) npart_tot += np;
-    __syncthreads();
-    np = 0;
-  };
-
-  int32_t cur = lane == 0 ? root : total;                  // this lane's range [cur, end)
-  int32_t end = total;
-  int sp = 0;                                              // donated ranges on the stack
-  for (;;) {
-    // idle lanes take ranges from the top of the stack
-    const bool idle = cur >= end;
-    const uint64_t im = __ballot(idle);
-    const int take = __popcll(im) < sp ? (int)__popcll(im) : sp;
-    if (idle) {
-      const int r = (int)__popcll(im & ltmask);
-      if (r < take) { cur = stk_b[sp - 1 - r]; end = stk_e[sp - 1 - r]; }
-    }
-    sp -= take;
-    const bool busy = cur < end;
-    if (!__ballot(busy)) break;
-    if (COUNT) ++niter;
-    if (nc > kPhysCells - kWave) flush_cells();
-
-    bool acc = false, bucket = false, donate = false;
-    int32_t cnt = 0, first = 0, nskip = 0;
-    double ncx = 0, ncy = 0, ncz = 0;
-    if (busy) {
-      const PhysNode nd = pn[cur];
-      ncx = nd.cx; ncy = nd.cy; ncz = nd.cz;
-      const float rx = (float)(ncx - gcx), ry = (float)(ncy - gcy), rz = (float)(ncz - gcz);
-      const float dx = fmaxf(fabsf(rx) - hx, 0.f), dy = fmaxf(fabsf(ry) - hy, 0.f), dz = fmaxf(fabsf(rz) - hz, 0.f);
-      const float d2 = dx * dx + dy * dy + dz * dz;
-      cnt = nd.count;
-      first = nd.first;
-      nskip = nd.skip > cur ? nd.skip : cur + 1;           // skip > cur by construction
-      if (d2 > nd.rc2) {
-        acc = true;
-        if (amag) {
-          const float b2 = nd.rc2 * th2;
-          const float d4 = d2 * d2;
-          acc = nd.m * nd.m * (b2 * b2 * b2) <= arel * (d4 * d4 * d2);
-        }
-      }
-      if (!acc) {
-        if (cnt <= nleaf) bucket = true;
-        else donate = nskip < end;                         // open; the rest of the range can go
-      }
-    }
-    // accepted cells -> cell list
-    const uint64_t am = __ballot(acc);
-    if (acc) {
-      const int pos = nc + (int)__popcll(am & ltmask);
-      const PhysNode& nd = pn[cur];
-      ca[pos] = V4<T>{(T)(ncx - gcx), (T)(ncy - gcy), (T)(ncz - gcz), (T)nd.m};
-      cb[pos] = V4<T>{(T)nd.qxx, (T)nd.qyy, (T)nd.qxy, (T)nd.qxz};
-      cq[pos] = (T)nd.qyz;
-    }
-    nc += (int)__popcll(am);
-    // opened nodes hand [skip, end) to the stack while it has room
-    const uint64_t dm = __ballot(donate);
-    const int room = kPhysStack - sp;
-    if (donate) {
-      const int r = (int)__popcll(dm & ltmask);
-      if (r < room) { stk_b[sp + r] = nskip; stk_e[sp + r] = end; end = nskip; }
-    }
-    sp += (int)__popcll(dm) < room ? (int)__popcll(dm) : room;
-    if (busy) cur = (acc || bucket) ? nskip : cur + 1;
-    if (COUNT) nopen += (uint32_t)__popcll(__ballot(busy && !acc && !bucket));
-
-    // buckets -> body list, all lanes loading bodies in parallel
-    const uint32_t cb_n = bucket ? (uint32_t)cnt : 0u;
-    const uint32_t bincl = wave_incl_scan_u32(cb_n);
-    const int nbody = (int)__shfl(bincl, kWave - 1, kWave);
-    const uint64_t bm = __ballot(bucket);
-    const int nb = (int)__popcll(bm);
-    if (bucket) {
-      const int j = (int)__popcll(bm & ltmask);
-      bfirst[j] = first;
-      bstart[j] = (int)(bincl - cb_n);
-    }
-    __syncthreads();
-    for (int base = 0; base < nbody; base += kWave) {
-      if (np > kPhysParts - kWave) flush_parts();
-      const int t = base + lane;
-      if (t < nbody) {
-        int lo_j = 0, hi_j = nb - 1;                       // last bucket with bstart <= t
-        while (lo_j < hi_j) {
-          const int mid = (lo_j + hi_j + 1) >> 1;
-          if (bstart[mid] <= t) lo_j = mid; else hi_j = mid - 1;
-        }
-        const int32_t b = bfirst[lo_j] + (t - bstart[lo_j]);
-        pl[np + lane] = V4<T>{(T)(st.x[b] - gcx), (T)(st.y[b] - gcy), (T)(st.z[b] - gcz), (T)st.m[b]};
-      }
-      np += nbody - base < kWave ? nbody - base : kWave;
-    }
-    __syncthreads();
-  }
-  if (nc) flush_cells();
-  if (np) flush_parts();
-
-  double v2 = 0.0;
-  if (valid) {
-    ax[i] = Ax; ay[i] = Ay; az[i] = Az;
-    if (POT) pot[i] = P + st.m[i] / eps;                   // drop the self pair, -m_i/eps
-    if (KICK) {
-      const double half = sy.dt[g.sys] * 0.5;
-      const double vx = st.vx[i] + Ax * half;
-      const double vy = st.vy[i] + Ay * half;
-      const double vz = st.vz[i] + Az * half;
-      st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
-      v2 = vx * vx + vy * vy + vz * vz;
-      if (!(v2 > 0.0)) v2 = 0.0;
-    }
-  }
-  if (KICK) {
-#pragma unroll
+    // group bounding box
+    double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
+  #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-      const double o = __shfl_xor(v2, off, kWave);
-      if (o > v2) v2 = o;
+  #pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double a = __shfl_xor(lo[k], off, kWave), b = __shfl_xor(hi[k], off, kWave);
+        lo[k] = a < lo[k] ? a : lo[k];
+        hi[k] = b > hi[k] ? b : hi[k];
+      }
     }
-    if (lane == 0 && v2 > 0.0) atomicMax(&sy.v2max[g.sys], (unsigned long long)__double_as_longlong(v2));
-  }
-  if (COUNT && lane == 0) {
-    const uint32_t nv = (uint32_t)g.count;
-    atomicAdd(&sc->n_pc, (unsigned long long)ncell_tot * nv);
-    atomicAdd(&sc->n_pp, (unsigned long long)npart_tot * nv);
-    atomicAdd(&sc->n_open, (unsigned long long)nopen);
-    atomicAdd(&sc->n_iter, (unsigned long long)niter);
+    const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]), gcz = 0.5 * (lo[2] + hi[2]);
+    const float hx = (float)(0.5 * (hi[0] - lo[0])) * (1.0f + 1e-6f);
+    const float hy = (float)(0.5 * (hi[1] - lo[1])) * (1.0f + 1e-6f);
+    const float hz = (float)(0.5 * (hi[2] - lo[2])) * (1.0f + 1e-6f);
+    const T tx = (T)(px - gcx), ty = (T)(py - gcy), tz = (T)(pz - gcz);
+    const T e2 = (T)(eps * eps);
+    // Relative criterion (amag != null): a cell that passes the geometric test
+    // is accepted only if its first neglected (octupole) term, ~ m bmax^3 / d^5,
+    // is at most alpha times the smallest previous |a| of the group:
+    // m^2 bmax^6 <= alpha^2 amin^2 d^10, with bmax^2 = rc2 * theta^2.
+    float arel = 0.f;
+    if (amag) {
+      float am = amag[ii];
+  #pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) am = fminf(am, __shfl_xor(am, off, kWave));
+      arel = rel2 * am * am;
+    }
+
+    double Ax = 0, Ay = 0, Az = 0, P = 0;
+    uint32_t ncell_tot = 0, npart_tot = 0, niter = 0, nopen = 0;
+    int nc = 0, np = 0;
+
+    auto flush_cells = [&]() {
+      __syncthreads();
+      T fx = 0, fy = 0, fz = 0, fp = 0;
+      for (int k = part; k < nc; k += parts) {
+        const V4<T> a = ca[k];
+        const V4<T> b = cb[k];
+        const T qyz = cq[k];
+        const T dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+        const T r2 = dx * dx + dy * dy + dz * dz + e2;
+        const T ri = rsq<T>(r2);
+        const T ri2 = ri * ri;
+        const T ri3 = ri2 * ri;
+        const T ri5 = ri3 * ri2;
+        const T qzz = -(b.x + b.y);
+        const T qdx = b.x * dx + b.z * dy + b.w * dz;
+        const T qdy = b.z * dx + b.y * dy + qyz * dz;
+        const T qdz = b.w * dx + qyz * dy + qzz * dz;
+        const T dqd = dx * qdx + dy * qdy + dz * qdz;
+        const T mono = a.w * ri3;
+        const T c7 = (T)2.5 * dqd * ri5 * ri2;
+        fx += (mono + c7) * dx - ri5 * qdx;
+        fy += (mono + c7) * dy - ri5 * qdy;
+        fz += (mono + c7) * dz - ri5 * qdz;
+        if (POT) fp -= a.w * ri + (T)0.5 * dqd * ri5;
+      }
+      combine_parts(fx, fy, fz, fp, width);
+      Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
+      if (POT) P += (double)fp;
+      if (COUNT) ncell_tot += nc;
+      __syncthreads();
+      nc = 0;
+    };
+    auto flush_parts = [&]() {
+      __syncthreads();
+      T fx = 0, fy = 0, fz = 0, fp = 0;
+      for (int k = part; k < np; k += parts) {
+        const V4<T> b = pl[k];
+        const T dx = b.x - tx, dy = b.y - ty, dz = b.z - tz;
+        const T r2 = dx * dx + dy * dy + dz * dz + e2;
+        const T ri = rsq<T>(r2);
+        const T mr3 = b.w * ri * ri * ri;
+        fx += mr3 * dx;
+        fy += mr3 * dy;
+        fz += mr3 * dz;
+        if (POT) fp -= b.w * ri;
+      }
+      combine_parts(fx, fy, fz, fp, width);
+      Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
+      if (POT) P += (double)fp;
+      if (COUNT) npart_tot += np;
+      __syncthreads();
+      np = 0;
+    };
+
+    int32_t cur = lane == 0 ? root : total;                  // this lane's range [cur, end)
+    int32_t end = total;
+    int sp = 0;                                              // donated ranges on the stack
+    for (;;) {
+      // idle lanes take ranges from the top of the stack
+      const bool idle = cur >= end;
+      const uint64_t im = __ballot(idle);
+      const int take = __popcll(im) < sp ? (int)__popcll(im) : sp;
+      if (idle) {
+        const int r = (int)__popcll(im & ltmask);
+        if (r < take) { cur = stk_b[sp - 1 - r]; end = stk_e[sp - 1 - r]; }
+      }
+      sp -= take;
+      const bool busy = cur < end;
+      if (!__ballot(busy)) break;
+      if (COUNT) ++niter;
+      if (nc > kPhysCells - kWave) flush_cells();
+
+      bool acc = false, bucket = false, donate = false;
+      int32_t cnt = 0, first = 0, nskip = 0;
+      double ncx = 0, ncy = 0, ncz = 0;
+      if (busy) {
+        const PhysNode nd = pn[cur];
+        ncx = nd.cx; ncy = nd.cy; ncz = nd.cz;
+        const float rx = (float)(ncx - gcx), ry = (float)(ncy - gcy), rz = (float)(ncz - gcz);
+        const float dx = fmaxf(fabsf(rx) - hx, 0.f), dy = fmaxf(fabsf(ry) - hy, 0.f), dz = fmaxf(fabsf(rz) - hz, 0.f);
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        cnt = nd.count;
+        first = nd.first;
+        nskip = nd.skip > cur ? nd.skip : cur + 1;           // skip > cur by construction
+        if (d2 > nd.rc2) {
+          acc = true;
+          if (amag) {
+            const float b2 = nd.rc2 * th2;
+            const float d4 = d2 * d2;
+            acc = nd.m * nd.m * (b2 * b2 * b2) <= arel * (d4 * d4 * d2);
+          }
+        }
+        if (!acc) {
+          if (cnt <= nleaf) bucket = true;
+          else donate = nskip < end;                         // open; the rest of the range can go
+        }
+      }
+      // accepted cells -> cell list
+      const uint64_t am = __ballot(acc);
+      if (acc) {
+        const int pos = nc + (int)__popcll(am & ltmask);
+        const PhysNode& nd = pn[cur];
+        ca[pos] = V4<T>{(T)(ncx - gcx), (T)(ncy - gcy), (T)(ncz - gcz), (T)nd.m};
+        cb[pos] = V4<T>{(T)nd.qxx, (T)nd.qyy, (T)nd.qxy, (T)nd.qxz};
+        cq[pos] = (T)nd.qyz;
+      }
+      nc += (int)__popcll(am);
+      // opened nodes hand [skip, end) to the stack while it has room
+      const uint64_t dm = __ballot(donate);
+      const int room = kPhysStack - sp;
+      if (donate) {
+        const int r = (int)__popcll(dm & ltmask);
+        if (r < room) { stk_b[sp + r] = nskip; stk_e[sp + r] = end; end = nskip; }
+      }
+      sp += (int)__popcll(dm) < room ? (int)__popcll(dm) : room;
+      if (busy) cur = (acc || bucket) ? nskip : cur + 1;
+      if (COUNT) nopen += (uint32_t)__popcll(__ballot(busy && !acc && !bucket));
+
+      // buckets -> body list, all lanes loading bodies in parallel
+      const uint32_t cb_n = bucket ? (uint32_t)cnt : 0u;
+      const uint32_t bincl = wave_incl_scan_u32(cb_n);
+      const int nbody = (int)__shfl(bincl, kWave - 1, kWave);
+      const uint64_t bm = __ballot(bucket);
+      const int nb = (int)__popcll(bm);
+      if (bucket) {
+        const int j = (int)__popcll(bm & ltmask);
+        bfirst[j] = first;
+        bstart[j] = (int)(bincl - cb_n);
+      }
+      __syncthreads();
+      for (int base = 0; base < nbody; base += kWave) {
+        if (np > kPhysParts - kWave) flush_parts();
+        const int t = base + lane;
+        if (t < nbody) {
+          int lo_j = 0, hi_j = nb - 1;                       // last bucket with bstart <= t
+          while (lo_j < hi_j) {
+            const int mid = (lo_j + hi_j + 1) >> 1;
+            if (bstart[mid] <= t) lo_j = mid; else hi_j = mid - 1;
+          }
+          const int32_t b = bfirst[lo_j] + (t - bstart[lo_j]);
+          pl[np + lane] = V4<T>{(T)(st.x[b] - gcx), (T)(st.y[b] - gcy), (T)(st.z[b] - gcz), (T)st.m[b]};
+        }
+        np += nbody - base < kWave ? nbody - base : kWave;
+      }
+      __syncthreads();
+    }
+    if (nc) flush_cells();
+    if (np) flush_parts();
+
+    double v2 = 0.0;
+    if (valid) {
+      ax[i] = Ax; ay[i] = Ay; az[i] = Az;
+      if (POT) pot[i] = P + st.m[i] / eps;                   // drop the self pair, -m_i/eps
+      if (KICK) {
+        const double half = sy.dt[g.sys] * 0.5;
+        const double vx = st.vx[i] + Ax * half;
+        const double vy = st.vy[i] + Ay * half;
+        const double vz = st.vz[i] + Az * half;
+        st.vx[i] = vx; st.vy[i] = vy; st.vz[i] = vz;
+        v2 = vx * vx + vy * vy + vz * vz;
+        if (!(v2 > 0.0)) v2 = 0.0;
+      }
+    }
+    if (KICK) {
+  #pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v2, off, kWave);
+        if (o > v2) v2 = o;
+      }
+      if (lane == 0 && v2 > 0.0) atomicMax(&sy.v2max[g.sys], (unsigned long long)__double_as_longlong(v2));
+    }
+    if (COUNT && lane == 0) {
+      const uint32_t nv = (uint32_t)g.count;
+      atomicAdd(&sc->n_pc, (unsigned long long)ncell_tot * nv);
+      atomicAdd(&sc->n_pp, (unsigned long long)npart_tot * nv);
+      atomicAdd(&sc->n_open, (unsigned long long)nopen);
+      atomicAdd(&sc->n_iter, (unsigned long long)niter);
+    }
+    __syncthreads();                                       // the next group reuses the LDS lists
   }
 }
 
@@ -618,12 +625,12 @@ __global__ __launch_bounds__(kBlock) void k_energy_final(const double* __restric
 }
 
 template <typename T>
-void walk_dispatch(unsigned nb, const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
-                   const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
-                   DevScalars* sc, bool kick, bool count, const float* amag, float rel2, float th2, hipStream_t s) {
-#define NBX_PW(K, P, C)                                                                                     \
-  hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, groups, ngroups, pn, st, sy, \
-                     eps, nleaf, ax, ay, az, pot, sc, amag, rel2, th2)
+void walk_dispatch(unsigned nb, const Group* groups, const PhysNode* pn, const StateArrays& st, const SysArrays& sy,
+                   double eps, int nleaf, double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick,
+                   bool count, const float* amag, float rel2, float th2, hipStream_t s) {
+#define NBX_PW(K, P, C)                                                                                          \
+  hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, groups, pn, st, sy, eps, nleaf, \
+                     ax, ay, az, pot, sc, amag, rel2, th2)
   const bool p = pot != nullptr;
   if (kick) {
     if (count) NBX_PW(true, false, true); else NBX_PW(true, false, false);
@@ -678,20 +685,18 @@ void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const
   hipLaunchKernelGGL(k_group_compact, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, gcnt, gmark, body_sys, groups);
 }
 
-void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
+void launch_phys_walk(const Group* groups, int64_t grid_groups, const PhysNode* pn, const StateArrays& st,
                       const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
                       DevScalars* sc, bool kick, bool count, bool fp64, const float* amag, double alpha,
                       double theta, hipStream_t s) {
-  if (ngroups <= 0) return;
-  const unsigned nb = (unsigned)(div_up(ngroups, 8) * 8);
+  if (grid_groups <= 0) return;
+  const unsigned nb = (unsigned)(div_up(grid_groups, 8) * 8);
   const float rel2 = (float)(alpha * alpha), th2 = (float)(theta * theta);
   if (alpha <= 0.0) amag = nullptr;
   if (fp64)
-    walk_dispatch<double>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
-                          th2, s);
+    walk_dispatch<double>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2, th2, s);
   else
-    walk_dispatch<float>(nb, groups, ngroups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
-                         th2, s);
+    walk_dispatch<float>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2, th2, s);
 }
 
 void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,

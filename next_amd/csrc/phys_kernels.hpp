@@ -70,8 +70,10 @@ void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const
 // the FP64 evaluation.  With amag (|a| of each body from the previous
 // evaluation, FP32) and alpha > 0, a cell that passes the geometric test
 // (bmax / theta) is accepted only if m bmax^3 <= alpha |a|min d^5, |a|min
-// the smallest |a| of the group and d the distance to the group's box.
-void launch_phys_walk(const Group* groups, int64_t ngroups, const PhysNode* pn, const StateArrays& st,
+// the smallest |a| of the group and d the distance to the group's box.  The
+// kernel takes the group count from sc->tree_groups; grid_groups (> 0) only
+// sizes the grid, and a grid below the count walks the rest in more rounds.
+void launch_phys_walk(const Group* groups, int64_t grid_groups, const PhysNode* pn, const StateArrays& st,
                       const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
                       DevScalars* sc, bool kick, bool count, bool fp64, const float* amag, double alpha,
                       double theta, hipStream_t s);
