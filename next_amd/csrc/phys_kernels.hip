@@ -330,9 +330,9 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
 
     // group bounding box
     double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
-  #pragma unroll
+#pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
-  #pragma unroll
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double a = __shfl_xor(lo[k], off, kWave), b = __shfl_xor(hi[k], off, kWave);
         lo[k] = a < lo[k] ? a : lo[k];
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
     float arel = 0.f;
     if (amag) {
       float am = amag[ii];
-  #pragma unroll
+#pragma unroll
       for (int off = 32; off >= 1; off >>= 1) am = fminf(am, __shfl_xor(am, off, kWave));
       arel = rel2 * am * am;
     }
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       }
     }
     if (KICK) {
-  #pragma unroll
+#pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
         const double o = __shfl_xor(v2, off, kWave);
         if (o > v2) v2 = o;
