@@ -483,6 +483,7 @@ def main() -> int:
                      "n_pp": cnt["n_pp"], "n_pc": cnt["n_pc"],
                      "valu_issue": ({"busy_frac": valu["derived"]["valu_busy_frac"],
                                      "instructions_per_slot": valu["derived"]["valu_instructions_per_slot"],
+                                     "salu_per_cu_cycle": valu["derived"]["salu_instructions_per_cu_cycle"],
                                      "source": "profiles/walk_valu.json"} if valu else None)},
         "interactions_per_s": (cnt["n_pp"] + cnt["n_pc"]) / (walk_ms * 1e-3) if walk_ms > 0 else 0.0,
         "walk_counters": {"n_open": cnt["n_open"], "traversal_wave_steps": cnt["n_iter"],
