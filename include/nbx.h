@@ -251,11 +251,12 @@ nbx_status nbx_debug_groups(nbx_ctx* ctx, int64_t cap, int32_t* first, int32_t* 
  * its own settings.  Requests are admitted first come, first served while
  * their estimated device memory fits the budget beside the admitted
  * requests' estimates, or beside what the batch contexts hold if that is
- * more (buffers grow with slack and are freed when a batch empties);
- * admitted requests with the
- * same settings share a batch context and step together, joining and leaving
- * between steps; batches with different settings overlap on the device.  A
- * request's result does not depend on the others. */
+ * more (buffers grow with slack and are freed when a batch empties).
+ * Admitted requests with the same settings share a batch context and step
+ * together, joining and leaving between steps; batches with different
+ * settings overlap on the device.  A request's result does not depend on the
+ * others.  Each distinct setting keeps its batch context (a stream and a
+ * host thread) until nbx_server_destroy. */
 
 typedef struct nbx_server nbx_server;
 
