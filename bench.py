@@ -56,38 +56,91 @@ def log(msg: str) -> None:
 
 
 def cpu_threads() -> int:
-    n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    """Host threads for NEXT's CPU code: the cores this process may use (its
+    affinity mask), capped by OMP_NUM_THREADS when the host sets it (the GPU
+    box allots a fixed share of its cores to each GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
 
-def cpu_baseline(p: ic.Particles, base_dt: float, theta: float, steps: int) -> dict:
+def host_cpu() -> dict:
+    """Model, sockets and core counts of this host (lscpu), and the cores
+    this process may use."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            key, _, val = line.partition(":")
+            info[key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    def num(key):
+        v = info.get(key, "")
+        return int(v) if v.isdigit() else None
+    sockets, per_socket, tpc = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"model": info.get("Model name"), "sockets": sockets, "cores_per_socket": per_socket,
+            "threads_per_core": tpc, "physical_cores": sockets * per_socket if sockets and per_socket else None,
+            "logical_cpus": os.cpu_count(), "usable_cpus": usable,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _harness_time(p: ic.Particles, base_dt: float, threads: int, warmup: int, steps: int) -> dict:
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "ic.bin")
+        with open(path, "wb") as f:
+            np.array([len(p)], np.int64).tofile(f)
+            np.concatenate([p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m]).astype(np.float64).tofile(f)
+            p.type.astype(np.int32).tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([harness, "time", path, str(threads), repr(base_dt), str(warmup), str(steps)],
+                           capture_output=True, text=True, env=env, check=True, timeout=600)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(p: ic.Particles, base_dt: float, theta: float, steps: int, c1: bool = True) -> dict:
     """NEXT's own Step (oracle/_ref/ref_harness, compiled from the reference
     headers) or, where that binary is absent, the oracle restatement, timed on
-    this host for `steps` steps of the same workload (no warm-up)."""
+    this host: 1 warm-up step and `steps` timed steps (median) of the same
+    workload, OpenMP over the cores this process may use; and C1 (Plummer
+    N = 16,384, base dt 1e-3) for all of its 100 steps."""
     threads = cpu_threads()
     harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    sample = f"{steps} KDK step(s) of the full N={len(p)} workload, theta={theta}, {threads} OpenMP threads"
+    sample = (f"1 warm-up + {steps} timed KDK steps (median) of the full N={len(p)} workload, theta={theta}, "
+              f"{threads} OpenMP threads")
+    host = host_cpu()
     if os.path.exists(harness) and theta == 0.5:
-        with tempfile.TemporaryDirectory() as td:
-            path = os.path.join(td, "ic.bin")
-            with open(path, "wb") as f:
-                np.array([len(p)], np.int64).tofile(f)
-                np.concatenate([p.x, p.y, p.z, p.vx, p.vy, p.vz, p.m]).astype(np.float64).tofile(f)
-                p.type.astype(np.int32).tofile(f)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-            r = subprocess.run([harness, "time", path, str(threads), repr(base_dt), "0", str(steps)],
-                               capture_output=True, text=True, env=env, check=True)
-            out = json.loads(r.stdout.strip().splitlines()[-1])
-        return {"value": out["ms_median"], "unit": "ms per KDK step", "cores": threads, "kind": "reference",
-                "sample": sample}
+        out = _harness_time(p, base_dt, threads, 1, steps)
+        res = {"value": out["ms_median"], "unit": "ms per KDK step", "cores": threads, "kind": "reference",
+               "sample": sample, "host": host}
+        if c1:
+            q = ic.plummer(16384, seed=1)
+            o1 = _harness_time(q, 1e-3, threads, 0, 100)
+            res["c1_plummer16k_100_steps"] = {"total_s": round(o1["ms_total"] / 1e3, 3),
+                                              "ms_per_step_median": o1["ms_median"], "threads": threads}
+        return res
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc  # the oracle: test infrastructure, used here only as the CPU baseline
 
     st = np.stack([p.x, p.y, p.z, p.vx, p.vy, p.vz])
-    t0 = time.perf_counter()
-    orc.run(st, p.m, p.type, base_dt, steps, theta=theta, threads=threads)
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    return {"value": ms, "unit": "ms per KDK step", "cores": threads, "kind": "port", "sample": sample}
+    st, _ = orc.run(st, p.m, p.type, base_dt, 1, theta=theta, threads=threads)   # warm-up
+    ms = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        st, _ = orc.run(st, p.m, p.type, base_dt, 1, theta=theta, threads=threads)
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return {"value": float(np.median(ms)), "unit": "ms per KDK step", "cores": threads, "kind": "port",
+            "sample": sample, "host": host}
 
 
 def ensemble(nsys: int, steps: int, warmup: int) -> dict:
@@ -509,8 +562,30 @@ def main() -> int:
             log("serving at %g/s: p50 %.1f ms" % (rate, out["serving"][-1]["p50_latency_ms"]))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.base_dt, args.theta, args.cpu_steps)
+    out["child_processes_at_exit"] = reap_children()
     print(json.dumps(out))
     return 0
+
+
+def reap_children() -> list:
+    """Every process this one started has ended: any child still present
+    (none is expected; the CPU baseline runs its harness to completion) is
+    named, terminated and waited for."""
+    try:
+        import psutil
+    except ImportError:
+        return []
+    left = psutil.Process().children(recursive=True)
+    names = [f"{c.pid}:{c.name()}" for c in left if c.is_running()]
+    for c in left:
+        try:
+            c.terminate()
+        except psutil.Error:
+            pass
+    psutil.wait_procs(left, timeout=10)
+    if names:
+        log("child processes left at exit (terminated): " + ", ".join(names))
+    return names
 
 
 if __name__ == "__main__":

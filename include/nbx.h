@@ -38,8 +38,12 @@ typedef enum nbx_status {
   NBX_OK = 0,
   NBX_ERR_ARG = -1,        /* bad argument: null pointer, n < 0, theta < 0 ... */
   NBX_ERR_NONFINITE = -2,  /* a position is NaN or infinite */
-  NBX_ERR_COINCIDENT = -3, /* two bodies share all 42 octree digits (NEXT's
-                              insertion recurses without end, octree.h:69-87) */
+  NBX_ERR_DEPTH = -3,      /* two bodies of one system share all 42 octree digits: the tree
+                              would need leaves deeper than 42 levels (cells below 2^-42 of the
+                              root's half-width).  NEXT's insertion (octree.h:69-87) keeps
+                              splitting for distinct bodies and recurses without end for
+                              coincident ones; this engine stops at 42 levels either way. */
+  NBX_ERR_COINCIDENT = NBX_ERR_DEPTH, /* former name of NBX_ERR_DEPTH */
   NBX_ERR_NOMEM = -4,      /* device allocation failed or budget exceeded */
   NBX_ERR_HIP = -5,        /* HIP runtime error */
   NBX_ERR_NODEVICE = -6,   /* no usable GPU */
@@ -66,7 +70,7 @@ typedef struct nbx_config {
   int device;        /* HIP device ordinal */
   int mode;          /* nbx_mode */
   int precision;     /* nbx_precision (NBX_MODE_PHYSICS only) */
-  double theta;      /* opening angle; NEXT fixes 0.5 (step.h:211) */
+  double theta;      /* opening angle; NEXT fixes 0.5 (step.h:32) */
   double eps;        /* Plummer softening length (NBX_MODE_PHYSICS only) */
   int leaf_size;     /* max bodies per leaf bucket, 1..64 (NBX_MODE_PHYSICS only) */
   int group_size;    /* at most this many targets per walk group, 1..64 (NBX_MODE_PHYSICS
@@ -171,7 +175,7 @@ int64_t nbx_system_size(const nbx_ctx* ctx, int32_t s);
 /* = computeAdaptiveDt (adaptive.h:23-52) on the device state. */
 nbx_status nbx_adaptive_dt(nbx_ctx* ctx, double base_dt, double* dt_out);
 
-/* = Step (step.h:204-383): one kick-drift-kick step of size dt (every system). */
+/* = Step (step.h:25-204): one kick-drift-kick step of size dt (every system). */
 nbx_status nbx_step(nbx_ctx* ctx, double dt);
 
 /* nsteps iterations of the driver loop body (begrun.cpp:97-99): dt from
@@ -218,10 +222,17 @@ nbx_status nbx_set_profiling(nbx_ctx* ctx, int flags);
 
 /* ---- parity hooks: the tree of the current positions ------------------ */
 
-/* Root centre and half-width (step.h:246-279) and each body's octree
+/* Root centre and half-width (step.h:93-102) and each body's octree
  * digits (octree.h:49-64): digits 0..20 in hi, 21..41 in lo, 3 bits each,
  * first digit most significant; upload order. */
 nbx_status nbx_debug_keys(nbx_ctx* ctx, double* root4, uint64_t* hi, uint64_t* lo);
+
+/* The body held in each device slot, in the engine's internal (key) order:
+ * its upload index within the context (id) and its system (sys), n entries
+ * each.  The ids of each system's slots form a permutation of that system's
+ * upload indices; the download calls check this and refuse anything else
+ * (NBX_ERR_STATE).  Test hook: no counterpart in NEXT. */
+nbx_status nbx_debug_order(nbx_ctx* ctx, int32_t* id, int32_t* sys);
 
 /* Number of tree nodes of the current positions. */
 nbx_status nbx_debug_tree_size(nbx_ctx* ctx, int64_t* nodes);
@@ -255,8 +266,11 @@ nbx_status nbx_debug_groups(nbx_ctx* ctx, int64_t cap, int32_t* first, int32_t* 
  * Admitted requests with the same settings share a batch context and step
  * together, joining and leaving between steps; batches with different
  * settings overlap on the device.  A request's result does not depend on the
- * others.  Each distinct setting keeps its batch context (a stream and a
- * host thread) until nbx_server_destroy. */
+ * others.  A batch context (a stream and a host thread) that has had no
+ * request for 2 seconds is retired, so a stream of ever new settings keeps
+ * only the batches it uses.  An engine failure inside a batch fails every
+ * request of that batch (NBX_ERR_* of the failing call) and resets it; no
+ * request receives another's bodies. */
 
 typedef struct nbx_server nbx_server;
 
@@ -288,8 +302,9 @@ typedef struct nbx_server_info {
 } nbx_server_info;
 
 nbx_status nbx_server_create(nbx_server** out, int device, uint64_t budget_bytes);
-/* Stops the server: requests not yet done are dropped.  Call it once no
- * other thread is inside nbx_server_wait. */
+/* Stops the server: requests not yet done fail with NBX_ERR_STATE, threads
+ * already inside nbx_server_wait on them return, and only then is the server
+ * freed.  No call on the server may start once destroy has been called. */
 void nbx_server_destroy(nbx_server* srv);
 nbx_status nbx_server_submit(nbx_server* srv, const nbx_request* req, int64_t* id);
 /* 0 queued, 1 running, 2 done, 3 failed; negative: unknown id. */

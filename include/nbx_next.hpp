@@ -68,7 +68,7 @@ class Session {
     return dt;
   }
 
-  // Step (step.h:204-383): one KDK step; positions and velocities are
+  // Step (step.h:25-204): one KDK step; positions and velocities are
   // written back into the caller's arrays.
   template <class PS>
   void step(PS& ps, double dt) {

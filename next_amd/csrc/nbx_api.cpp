@@ -2,7 +2,7 @@
 // host-side orchestration of one KDK step on one HIP stream.
 //
 // A context holds one or more independent particle systems, stored one after
-// another.  One step of all of them (step.h:204-383, with the second tree of
+// another.  One step of all of them (step.h:25-204, with the second tree of
 // step n reused as the first tree of step n+1: the walk is deterministic, so
 // a(x_{n+1}) from the closing kick of step n is bit-identical to the
 // opening-kick force of step n+1):
@@ -77,6 +77,8 @@ struct nbx_ctx {
   int nboxes = 0;
   DevScalars* sc = nullptr;   // device
   DevScalars* hsc = nullptr;  // pinned host mirror
+  void* hstage = nullptr;     // pinned host staging for device-to-host copies
+  size_t hstage_bytes = 0;
   bool physics = false;       // NBX_MODE_PHYSICS
   double alpha = 0.0;         // physics mode: relative opening criterion (0: geometric only)
   DBuf amag[2];               // |a| per body from the opening kick, in key order (FP32)
@@ -438,8 +440,9 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
     NBXCHK(sync_scalars(c));
     if (c->hsc->coincident > 0)
-      return fail(c, NBX_ERR_COINCIDENT,
-                  std::to_string(c->hsc->coincident) + " pair(s) of bodies share all 42 octree digits");
+      return fail(c, NBX_ERR_DEPTH,
+                  std::to_string(c->hsc->coincident) +
+                      " pair(s) of bodies share all 42 octree digits (the tree is limited to 42 levels)");
   }
   c->skey = skey;
   c->skey2 = skey2;
@@ -664,29 +667,55 @@ int64_t sys_offset(const nbx_ctx* c, int s) {
   return o;
 }
 
-// Copy bodies [off, off + cnt) of a key-order device array out, each placed
-// at its upload index minus `base`.
-nbx_status download_range(nbx_ctx* c, const void* dev, size_t esz, int64_t off, int64_t cnt, int64_t base,
-                          const std::vector<int32_t>& id, void* out) {
-  std::vector<char> tmp((size_t)cnt * esz);
-  HIPCHK(c, hipMemcpyAsync(tmp.data(), static_cast<const char*>(dev) + off * esz, (size_t)cnt * esz,
-                           hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  char* o = static_cast<char*>(out);
-  for (int64_t k = 0; k < cnt; ++k) {
-    const int64_t at = (int64_t)id[(size_t)k] - base;
-    if (at < 0 || at >= cnt) return fail(c, NBX_ERR_STATE, "inconsistent body indices on the device");
-    std::memcpy(o + at * esz, &tmp[(size_t)k * esz], esz);
+// Device-to-host copy through the context's pinned staging buffer: the
+// stream is drained before the host reads, whatever the runtime does with
+// copies into pageable memory.  Returns the staged bytes.
+nbx_status fetch(nbx_ctx* c, const void* dev, size_t bytes, const void** staged) {
+  if (bytes > c->hstage_bytes) {
+    if (c->hstage) (void)hipHostFree(c->hstage);
+    c->hstage = nullptr;
+    c->hstage_bytes = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
+    HIPCHK(c, hipHostMalloc(&c->hstage, want, hipHostMallocDefault));
+    c->hstage_bytes = want;
   }
+  if (bytes) HIPCHK(c, hipMemcpyAsync(c->hstage, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *staged = c->hstage;
   return NBX_OK;
 }
 
-// Upload indices (`id`) of the bodies in slots [off, off + cnt).
-nbx_status read_ids(nbx_ctx* c, int64_t off, int64_t cnt, std::vector<int32_t>& id) {
-  id.resize((size_t)cnt);
-  HIPCHK(c, hipMemcpyAsync(id.data(), P<int32_t>(c->st[c->cur][8]) + off, (size_t)cnt * 4, hipMemcpyDeviceToHost,
-                           c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+// Copy bodies [off, off + cnt) of a key-order device array out, each placed
+// at its upload index minus `base` (`id` from read_ids, checked there to be a
+// permutation of [base, base + cnt)).
+nbx_status download_range(nbx_ctx* c, const void* dev, size_t esz, int64_t off, int64_t cnt, int64_t base,
+                          const std::vector<int32_t>& id, void* out) {
+  const void* staged = nullptr;
+  NBXCHK(fetch(c, static_cast<const char*>(dev) + off * esz, (size_t)cnt * esz, &staged));
+  const char* src = static_cast<const char*>(staged);
+  char* o = static_cast<char*>(out);
+  for (int64_t k = 0; k < cnt; ++k)
+    std::memcpy(o + ((int64_t)id[(size_t)k] - base) * esz, src + (size_t)k * esz, esz);
+  return NBX_OK;
+}
+
+// Upload indices (`id`) of the bodies in slots [off, off + cnt), which must
+// be a permutation of [base, base + cnt): every later scatter by upload index
+// relies on it, so anything else is refused (NBX_ERR_STATE) before a byte is
+// written to the caller's arrays.
+nbx_status read_ids(nbx_ctx* c, int64_t off, int64_t cnt, int64_t base, std::vector<int32_t>& id) {
+  const void* staged = nullptr;
+  NBXCHK(fetch(c, P<int32_t>(c->st[c->cur][8]) + off, (size_t)cnt * 4, &staged));
+  id.assign(static_cast<const int32_t*>(staged), static_cast<const int32_t*>(staged) + cnt);
+  std::vector<uint8_t> seen((size_t)cnt, 0);
+  for (int64_t k = 0; k < cnt; ++k) {
+    const int64_t at = (int64_t)id[(size_t)k] - base;
+    if (at < 0 || at >= cnt || seen[(size_t)at])
+      return fail(c, NBX_ERR_STATE,
+                  "body indices on the device are not a permutation (slot " + std::to_string(off + k) + " holds " +
+                      std::to_string(id[(size_t)k]) + ")");
+    seen[(size_t)at] = 1;
+  }
   return NBX_OK;
 }
 
@@ -783,6 +812,7 @@ void nbx_destroy(nbx_ctx* c) {
   if (c->ev_scalars) (void)hipEventDestroy(c->ev_scalars);
   if (c->sc) (void)hipFree(c->sc);
   if (c->hsc) (void)hipHostFree(c->hsc);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -955,7 +985,7 @@ nbx_status nbx_download_system(nbx_ctx* c, int32_t s, double* x, double* y, doub
   if (s < 0 || s >= (int32_t)c->sys_n.size()) return fail(c, NBX_ERR_ARG, "no such system");
   const int64_t off = sys_offset(c, s), cnt = c->sys_n[(size_t)s];
   std::vector<int32_t> id;
-  NBXCHK(read_ids(c, off, cnt, id));
+  NBXCHK(read_ids(c, off, cnt, off, id));
   double* outs[6] = {x, y, z, vx, vy, vz};
   for (int k = 0; k < 6; ++k)
     if (outs[k]) NBXCHK(download_range(c, c->st[c->cur][k].p, 8, off, cnt, off, id, outs[k]));
@@ -980,13 +1010,13 @@ nbx_status nbx_remove_system(nbx_ctx* c, int32_t s, double* x, double* y, double
   }
   launch_relabel(n_new, P<int32_t>(c->st[c->cur][9]), s, st);
   // upload indices of later systems move down by cnt
-  std::vector<int32_t> ids((size_t)n_new);
   if (n_new > 0) {
-    HIPCHK(c, hipMemcpyAsync(ids.data(), c->st[c->cur][8].p, (size_t)n_new * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    for (auto& v : ids)
-      if (v >= off + cnt) v -= (int32_t)cnt;
-    HIPCHK(c, hipMemcpyAsync(c->st[c->cur][8].p, ids.data(), (size_t)n_new * 4, hipMemcpyHostToDevice, st));
+    const void* staged = nullptr;
+    NBXCHK(fetch(c, c->st[c->cur][8].p, (size_t)n_new * 4, &staged));
+    int32_t* ids = static_cast<int32_t*>(const_cast<void*>(staged));
+    for (int64_t k = 0; k < n_new; ++k)
+      if (ids[k] >= off + cnt) ids[k] -= (int32_t)cnt;
+    HIPCHK(c, hipMemcpyAsync(c->st[c->cur][8].p, ids, (size_t)n_new * 4, hipMemcpyHostToDevice, st));
   }
   HIPCHK(c, hipStreamSynchronize(st));
   const bool kept = c->acc_valid;
@@ -1002,7 +1032,7 @@ nbx_status nbx_download(nbx_ctx* c, double* x, double* y, double* z, double* vx,
   const int64_t n = c->n;
   if (n == 0) return NBX_OK;
   std::vector<int32_t> id;
-  NBXCHK(read_ids(c, 0, n, id));
+  NBXCHK(read_ids(c, 0, n, 0, id));
   double* outs[6] = {x, y, z, vx, vy, vz};
   for (int k = 0; k < 6; ++k)
     if (outs[k]) NBXCHK(download_range(c, c->st[c->cur][k].p, 8, 0, n, 0, id, outs[k]));
@@ -1065,7 +1095,7 @@ nbx_status nbx_accel(nbx_ctx* c, double* ax, double* ay, double* az) {
   NBXCHK(ensure_accel(c));
   if (fresh) NBXCHK(finish_timing(c, false));
   std::vector<int32_t> id;
-  NBXCHK(read_ids(c, 0, c->n, id));
+  NBXCHK(read_ids(c, 0, c->n, 0, id));
   double* outs[3] = {ax, ay, az};
   for (int k = 0; k < 3; ++k) NBXCHK(download_range(c, c->acc[k].p, 8, 0, c->n, 0, id, outs[k]));
   return NBX_OK;
@@ -1073,7 +1103,7 @@ nbx_status nbx_accel(nbx_ctx* c, double* ax, double* ay, double* az) {
 
 static nbx_status download_unpermuted(nbx_ctx* c, const void* dev, double* out) {
   std::vector<int32_t> id;
-  NBXCHK(read_ids(c, 0, c->n, id));
+  NBXCHK(read_ids(c, 0, c->n, 0, id));
   return download_range(c, dev, 8, 0, c->n, 0, id, out);
 }
 
@@ -1198,6 +1228,18 @@ nbx_status nbx_debug_keys(nbx_ctx* c, double* root4, uint64_t* hi, uint64_t* lo)
   c->skey2 = nullptr;  // the second-word buffers were reused
   c->tree_valid = false;
   c->layout_valid = false;
+  return NBX_OK;
+}
+
+nbx_status nbx_debug_order(nbx_ctx* c, int32_t* id, int32_t* sys) {
+  NBXCHK(check_ctx(c));
+  if (c->n == 0) return NBX_OK;
+  if (!(id && sys)) return fail(c, NBX_ERR_ARG, "null output");
+  const void* staged = nullptr;
+  NBXCHK(fetch(c, c->st[c->cur][8].p, (size_t)c->n * 4, &staged));
+  std::memcpy(id, staged, (size_t)c->n * 4);
+  NBXCHK(fetch(c, c->st[c->cur][9].p, (size_t)c->n * 4, &staged));
+  std::memcpy(sys, staged, (size_t)c->n * 4);
   return NBX_OK;
 }
 

@@ -11,7 +11,13 @@
 // run on their own streams, so their work overlaps on the device.  A
 // system's trajectory does not depend on the other systems of its batch, so
 // a request's result is the same alone and in any mix.  A batch that empties
-// frees its device buffers, so only batches with work hold memory.
+// frees its device buffers, so only batches with work hold memory, and a
+// batch left without work for kRetireIdle is retired altogether (its thread,
+// stream and context end), so a stream of requests with ever new settings
+// keeps only the batches it uses.  An engine call that fails mid-batch fails
+// every request of that batch and resets its context: no request can receive
+// another's bodies.  The engine is reached through a table of calls
+// (nbx_serve_ops.h), this library's own by default.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -26,6 +32,7 @@
 #include <vector>
 
 #include "nbx.h"
+#include "nbx_serve_ops.h"
 
 namespace {
 
@@ -37,6 +44,8 @@ double ms_since(Clock::time_point t0, Clock::time_point t1) {
 
 // Estimated device bytes per body (state, keys, tree nodes of both modes).
 constexpr size_t kBytesPerBody = 700;
+// A batch without members or newcomers for this long is retired.
+constexpr auto kRetireIdle = std::chrono::milliseconds(2000);
 
 struct Request {
   int64_t id = 0;
@@ -72,11 +81,13 @@ struct Batch {
   std::deque<Request*> incoming;
   std::thread thread;
   std::atomic<uint64_t> held{0};    // device bytes of ctx after its last step
+  std::atomic<bool> ended{false};   // retired: its thread has released the context and is returning
 };
 
 }  // namespace
 
 struct nbx_server {
+  nbx_engine_ops ops{};
   int device = 0;
   size_t budget = 0;
   size_t in_use = 0;
@@ -85,6 +96,8 @@ struct nbx_server {
   std::deque<Request*> pending;
   std::map<int64_t, std::unique_ptr<Request>> requests;
   std::map<BatchKey, std::unique_ptr<Batch>> batches;
+  std::vector<std::unique_ptr<Batch>> retired;   // threads still to be joined
+  int64_t waiters = 0;              // threads inside nbx_server_wait
   int64_t next_id = 1;
   bool stop = false;
   std::thread scheduler;
@@ -94,12 +107,27 @@ struct nbx_server {
 
 namespace {
 
-void finish(nbx_server* s, Request* r, nbx_status st, const std::string& err) {
+// Caller holds the lock.  `admitted`: the request's bytes are counted in
+// in_use (it left the queue).
+void finish(nbx_server* s, Request* r, nbx_status st, const std::string& err, bool admitted = true) {
   r->status = st;
   r->error = err;
   r->state = st == NBX_OK ? 2 : 3;
   r->t_done = Clock::now();
-  s->in_use -= r->bytes;
+  if (admitted) s->in_use -= r->bytes;
+}
+
+// An engine call failed: the context's systems no longer match the
+// members, so every member fails with that error and the context is reset.
+void fail_batch(nbx_server* s, Batch* b, nbx_status st, const std::string& err) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Request* r : b->members) finish(s, r, st, err);
+    b->members.clear();
+  }
+  (void)s->ops.upload_batch(b->ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr);
+  s->cv.notify_all();
 }
 
 // One batch: admit its newcomers, step everyone once, retire the finished.
@@ -112,9 +140,9 @@ void settle(nbx_server* s, Batch* b) {
       std::lock_guard<std::mutex> lk(s->mu);
       idle = b->incoming.empty();
     }
-    if (idle) (void)nbx_trim(b->ctx);   // on failure the buffers stay and are reused
+    if (idle) (void)s->ops.trim(b->ctx);   // on failure the buffers stay and are reused
   }
-  const uint64_t bytes = nbx_device_bytes(b->ctx);
+  const uint64_t bytes = s->ops.device_bytes(b->ctx);
   {
     std::lock_guard<std::mutex> lk(s->mu);   // the scheduler admits against it
     b->held = bytes;
@@ -127,18 +155,46 @@ void batch_loop(nbx_server* s, Batch* b) {
     std::vector<Request*> joining;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&] { return s->stop || !b->incoming.empty() || !b->members.empty(); });
+      const bool work = s->cv.wait_for(lk, kRetireIdle, [&] {
+        return s->stop || !b->incoming.empty() || !b->members.empty();
+      });
       if (s->stop) return;
+      if (!work) {
+        // Idle for kRetireIdle: leave the map under the lock, so the
+        // scheduler either queued a newcomer before (and this batch stays)
+        // or finds no batch for these settings and makes a new one.
+        auto it = s->batches.find(b->key);
+        if (it != s->batches.end() && it->second.get() == b) {
+          s->retired.push_back(std::move(it->second));
+          s->batches.erase(it);
+          lk.unlock();
+          s->ops.destroy(b->ctx);
+          b->ctx = nullptr;
+          b->ended = true;
+          s->cv.notify_all();   // the scheduler joins this thread
+          return;
+        }
+        continue;
+      }
       joining.assign(b->incoming.begin(), b->incoming.end());
       b->incoming.clear();
     }
     for (Request* r : joining) {
       int32_t sys = -1;
-      const nbx_status st = nbx_add_system(b->ctx, r->spec.n, r->x.data(), r->y.data(), r->z.data(), r->vx.data(),
-                                           r->vy.data(), r->vz.data(), r->m.data(), r->type.data(), &sys);
+      const nbx_status st = s->ops.add_system(b->ctx, r->spec.n, r->x.data(), r->y.data(), r->z.data(), r->vx.data(),
+                                              r->vy.data(), r->vz.data(), r->m.data(), r->type.data(), &sys);
+      if (st == NBX_OK && sys != (int32_t)b->members.size()) {
+        // the context's systems and the members no longer correspond
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          finish(s, r, NBX_ERR_STATE, "the batch context numbered a new system out of order");
+        }
+        fail_batch(s, b, NBX_ERR_STATE, "the batch context numbered a new system out of order");
+        continue;
+      }
       std::lock_guard<std::mutex> lk(s->mu);
       if (st != NBX_OK) {
-        finish(s, r, st, nbx_last_error(b->ctx));
+        finish(s, r, st, s->ops.last_error(b->ctx));
         s->cv.notify_all();
         continue;
       }
@@ -146,18 +202,15 @@ void batch_loop(nbx_server* s, Batch* b) {
       r->t_start = Clock::now();
       b->members.push_back(r);
     }
-    if (b->members.empty()) continue;
+    if (b->members.empty()) {
+      settle(s, b);
+      continue;
+    }
     int64_t bodies = 0;
     for (Request* r : b->members) bodies += r->spec.n;
-    const nbx_status st = nbx_run(b->ctx, b->key.dt, 1, nullptr);
+    const nbx_status st = s->ops.run(b->ctx, b->key.dt, 1, nullptr);
     if (st != NBX_OK) {
-      const std::string err = nbx_last_error(b->ctx);
-      std::lock_guard<std::mutex> lk(s->mu);
-      for (Request* r : b->members) finish(s, r, st, err);
-      b->members.clear();
-      (void)nbx_upload_batch(b->ctx, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                             nullptr);
-      s->cv.notify_all();
+      fail_batch(s, b, st, s->ops.last_error(b->ctx));
       settle(s, b);
       continue;
     }
@@ -168,14 +221,33 @@ void batch_loop(nbx_server* s, Batch* b) {
     for (int32_t k = (int32_t)b->members.size() - 1; k >= 0; --k) {
       Request* r = b->members[(size_t)k];
       if (r->steps_done < r->spec.nsteps) continue;
-      const nbx_status rs = nbx_remove_system(b->ctx, k, r->x.data(), r->y.data(), r->z.data(), r->vx.data(),
-                                              r->vy.data(), r->vz.data());
+      const nbx_status rs = s->ops.remove_system(b->ctx, k, r->x.data(), r->y.data(), r->z.data(), r->vx.data(),
+                                                 r->vy.data(), r->vz.data());
+      if (rs != NBX_OK) {
+        // The context may still hold the system, or hold the others in a
+        // changed order: fail the whole batch (this request included).
+        fail_batch(s, b, rs, s->ops.last_error(b->ctx));
+        break;
+      }
       std::lock_guard<std::mutex> lk(s->mu);
-      finish(s, r, rs, rs == NBX_OK ? std::string() : std::string(nbx_last_error(b->ctx)));
+      finish(s, r, NBX_OK, std::string());
       b->members.erase(b->members.begin() + k);
       s->cv.notify_all();
     }
     settle(s, b);
+  }
+}
+
+// Join the threads of retired batches that have ended (caller holds the
+// lock; an ended thread takes the lock no more).
+void reap(nbx_server* s) {
+  for (size_t k = 0; k < s->retired.size();) {
+    if (s->retired[k]->ended) {
+      if (s->retired[k]->thread.joinable()) s->retired[k]->thread.join();
+      s->retired.erase(s->retired.begin() + (long)k);
+    } else {
+      ++k;
+    }
   }
 }
 
@@ -192,9 +264,15 @@ size_t committed(const nbx_server* s) {
 void scheduler_loop(nbx_server* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   auto fits = [&] { return !s->pending.empty() && committed(s) + s->pending.front()->bytes <= s->budget; };
+  auto ended = [&] {
+    for (const auto& b : s->retired)
+      if (b->ended) return true;
+    return false;
+  };
   for (;;) {
-    s->cv.wait(lk, [&] { return s->stop || fits(); });
+    s->cv.wait(lk, [&] { return s->stop || fits() || ended(); });
     if (s->stop) return;
+    reap(s);
     while (fits()) {
       Request* r = s->pending.front();
       s->pending.pop_front();
@@ -213,7 +291,7 @@ void scheduler_loop(nbx_server* s) {
         cfg.eps = key.eps;
         cfg.leaf_size = key.leaf;
         cfg.adaptive_dt = key.adaptive;
-        const nbx_status st = nbx_create(&b->ctx, &cfg);
+        const nbx_status st = s->ops.create(&b->ctx, &cfg);
         if (st != NBX_OK) {
           s->in_use += r->bytes;
           finish(s, r, st, "could not create a context for this request's settings");
@@ -235,14 +313,25 @@ void scheduler_loop(nbx_server* s) {
 
 extern "C" {
 
-nbx_status nbx_server_create(nbx_server** out, int device, uint64_t budget_bytes) {
-  if (!out || budget_bytes == 0) return NBX_ERR_ARG;
+nbx_status nbx_server_create_with_ops(nbx_server** out, int device, uint64_t budget_bytes,
+                                      const nbx_engine_ops* ops) {
+  if (!out || budget_bytes == 0 || !ops) return NBX_ERR_ARG;
+  if (!(ops->create && ops->destroy && ops->last_error && ops->add_system && ops->remove_system && ops->run &&
+        ops->upload_batch && ops->trim && ops->device_bytes))
+    return NBX_ERR_ARG;
   nbx_server* s = new nbx_server;
+  s->ops = *ops;
   s->device = device;
   s->budget = (size_t)budget_bytes;
   s->scheduler = std::thread(scheduler_loop, s);
   *out = s;
   return NBX_OK;
+}
+
+nbx_status nbx_server_create(nbx_server** out, int device, uint64_t budget_bytes) {
+  const nbx_engine_ops ops{nbx_create,     nbx_destroy,       nbx_last_error, nbx_add_system, nbx_remove_system,
+                           nbx_run,        nbx_upload_batch,  nbx_trim,       nbx_device_bytes};
+  return nbx_server_create_with_ops(out, device, budget_bytes, &ops);
 }
 
 void nbx_server_destroy(nbx_server* s) {
@@ -253,10 +342,23 @@ void nbx_server_destroy(nbx_server* s) {
   }
   s->cv.notify_all();
   if (s->scheduler.joinable()) s->scheduler.join();
-  for (auto& kv : s->batches) {
+  for (auto& kv : s->batches)
     if (kv.second->thread.joinable()) kv.second->thread.join();
-    nbx_destroy(kv.second->ctx);
+  for (auto& b : s->retired)
+    if (b->thread.joinable()) b->thread.join();
+  {
+    // Requests not done fail, and threads already waiting on them return
+    // before the server goes away.
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (Request* r : s->pending) finish(s, r, NBX_ERR_STATE, "the server was stopped", false);
+    s->pending.clear();
+    for (auto& kv : s->requests)
+      if (kv.second->state < 2) finish(s, kv.second.get(), NBX_ERR_STATE, "the server was stopped");
+    s->cv.notify_all();
+    s->cv.wait(lk, [&] { return s->waiters == 0; });
   }
+  for (auto& kv : s->batches)
+    if (kv.second->ctx) s->ops.destroy(kv.second->ctx);
   delete s;
 }
 
@@ -305,6 +407,15 @@ nbx_status nbx_server_wait(nbx_server* s, int64_t id, double* x, double* y, doub
                            double* vz, nbx_request_timing* timing) {
   if (!s) return NBX_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
+  // nbx_server_destroy waits for every thread in here to leave.
+  ++s->waiters;
+  struct Leave {
+    nbx_server* s;
+    ~Leave() {
+      --s->waiters;
+      s->cv.notify_all();
+    }
+  } leave{s};
   // Look the id up again on every wake-up: another thread waiting on the
   // same id may have received the result and released the request.
   auto it = s->requests.end();

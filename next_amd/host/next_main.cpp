@@ -13,7 +13,7 @@
 // Options (after the positional arguments):
 //   --steps N        stop after N steps
 //   --t-end T        stop once the simulated time reaches T
-//   --theta X        opening angle (NEXT: 0.5, step.h:211)
+//   --theta X        opening angle (NEXT: 0.5, step.h:32)
 //   --fixed-dt       use dt as given instead of the speed-adaptive step
 //   --device D       HIP device ordinal
 //   --quiet-dumps    run without writing snapshots (timing runs)
@@ -136,14 +136,16 @@ int main(int argc, char** argv) {
     return 1;
   }
 #endif
-  static const char* kBanner = R"(
- _  _ ________   _________
-| \ | |  ____\ \ / /__   __|
-|  \| | |__   \ V /   | |
-| . ` |  __|   > <    | |
-| |\  | |____ / . \   | |
-|_| \_|______/_/ \_\  |_|
-)";
+  // begrun.cpp:60-67, trailing spaces included: a raw string that starts
+  // and ends with a newline, printed with std::endl.
+  static const char* kBanner =
+      "\n"
+      " _  _ ________   _________ \n"
+      "| \\ | |  ____\\ \\ / /__   __|\n"
+      "|  \\| | |__   \\ V /   | |   \n"
+      "| . ` |  __|   > <    | |   \n"
+      "| |\\  | |____ / . \\   | |   \n"
+      "|_| \\_|______/_/ \\_\\  |_|   \n";
   std::cout << kBanner << std::endl;
   std::cout << " Threads:   " << args.threads << std::endl;
   std::cout << " Precision: FP64" << std::endl;

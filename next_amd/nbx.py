@@ -16,7 +16,7 @@ from .ic import Particles
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnbx.so")
 
-STATUS = {0: "OK", -1: "NBX_ERR_ARG", -2: "NBX_ERR_NONFINITE", -3: "NBX_ERR_COINCIDENT",
+STATUS = {0: "OK", -1: "NBX_ERR_ARG", -2: "NBX_ERR_NONFINITE", -3: "NBX_ERR_DEPTH",
           -4: "NBX_ERR_NOMEM", -5: "NBX_ERR_HIP", -6: "NBX_ERR_NODEVICE", -7: "NBX_ERR_STATE"}
 
 MODE_REFERENCE, MODE_PHYSICS = 0, 1
@@ -27,10 +27,11 @@ EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", 
            "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
            "nbx_reset_timing", "nbx_set_profiling", "nbx_debug_keys", "nbx_debug_tree_size", "nbx_debug_tree",
-           "nbx_debug_groups", "nbx_autotune",
+           "nbx_debug_groups", "nbx_debug_order", "nbx_autotune",
            "nbx_potential", "nbx_direct", "nbx_energy", "nbx_energy_systems", "nbx_upload_batch", "nbx_add_system",
            "nbx_download_system", "nbx_remove_system", "nbx_num_systems", "nbx_system_size", "nbx_server_create",
-           "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats"]
+           "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats",
+           "nbx_server_create_with_ops"]
 
 
 class NbxError(RuntimeError):
@@ -110,6 +111,7 @@ def lib():
     L.nbx_debug_tree_size.argtypes = [VP, C.POINTER(C.c_int64)]
     L.nbx_debug_tree.argtypes = [VP, i32p, i32p, i32p, i32p, f64p, f64p, f64p, f64p, i32p]
     L.nbx_debug_groups.argtypes = [VP, C.c_int64, i32p, i32p, i32p, C.POINTER(C.c_int64)]
+    L.nbx_debug_order.argtypes = [VP, i32p, i32p]
     L.nbx_autotune.argtypes = [VP, i32p, i32p, C.c_int32, C.c_double, C.c_int32, f64p]
     L.nbx_potential.argtypes = [VP, f64p]
     L.nbx_direct.argtypes = [VP, VP, VP, VP, VP]
@@ -323,6 +325,13 @@ class Engine:
         out = {key: v[:k] for key, v in t.items() if key != "perm"}
         out["perm"] = t["perm"][: self.n]
         return out
+
+    def debug_order(self):
+        """(upload index, system) of the body in each device slot (key order)."""
+        n = max(self.n, 1)
+        ids, sys_ = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        self._check(lib().nbx_debug_order(self._h, ids, sys_))
+        return ids[: self.n], sys_[: self.n]
 
     def autotune(self, candidates, dt: float, steps: int = 2):
         """Time each (leaf_size, group_size) candidate on the current systems

@@ -24,6 +24,7 @@
 //   orc_step/orc_run  golden multi-step state and dt sequence
 //   orc_direct        Plummer-softened direct sum: two-body closed form and
 //                     Newton's third law (total force ~ 0)
+//   orc_direct_subset equality with orc_direct on the same targets
 #include <omp.h>
 
 #include <algorithm>
@@ -153,7 +154,7 @@ void upward(Tree& t, int id, const double* x, const double* y, const double* z, 
   p.qxx = qxx; p.qyy = qyy; p.qzz = qzz; p.qxy = qxy; p.qxz = qxz; p.qyz = qyz;
 }
 
-// Root geometry of the tree-build lambda (step.h:246-279).
+// Root geometry of the tree-build lambda (step.h:68-102).
 void root_geometry(int64_t n, const double* x, const double* y, const double* z, double out[4]) {
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
   for (int64_t i = 0; i < n; ++i) {
@@ -345,7 +346,7 @@ double orc_adaptive_dt(int64_t n, const double* vx, const double* vy, const doub
   return dt;
 }
 
-// One KDK step with the reference's structure (step.h:204-367, non-MPI):
+// One KDK step with the reference's structure (step.h:25-204, non-MPI):
 // build + walk + half kick, drift, build + walk + half kick.
 // Returns 0, or 1 if a build hit the depth cap (coincident bodies).
 int orc_step(int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
@@ -385,23 +386,47 @@ int orc_run(int64_t n, double* x, double* y, double* z, double* vx, double* vy, 
 
 // FP64 direct sum with Plummer softening eps (G = 1): the force reference for
 // the mixed-precision tree.  pot may be null.
+static void direct_one(int64_t n, const double* x, const double* y, const double* z, const double* m, double e2,
+                       int64_t i, double* out4) {
+  double a0 = 0, a1 = 0, a2 = 0, p = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    if (j == i) continue;
+    double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+    double r2 = dx * dx + dy * dy + dz * dz + e2;
+    double rinv = 1.0 / std::sqrt(r2);
+    double mr3 = m[j] * rinv * rinv * rinv;
+    a0 += dx * mr3; a1 += dy * mr3; a2 += dz * mr3;
+    p -= m[j] * rinv;
+  }
+  out4[0] = a0; out4[1] = a1; out4[2] = a2; out4[3] = p;
+}
+
 void orc_direct(int64_t n, const double* x, const double* y, const double* z, const double* m,
                 double eps, double* ax, double* ay, double* az, double* pot, int threads) {
   const double e2 = eps * eps;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
   for (int64_t i = 0; i < n; ++i) {
-    double a0 = 0, a1 = 0, a2 = 0, p = 0;
-    for (int64_t j = 0; j < n; ++j) {
-      if (j == i) continue;
-      double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
-      double r2 = dx * dx + dy * dy + dz * dz + e2;
-      double rinv = 1.0 / std::sqrt(r2);
-      double mr3 = m[j] * rinv * rinv * rinv;
-      a0 += dx * mr3; a1 += dy * mr3; a2 += dz * mr3;
-      p -= m[j] * rinv;
-    }
-    ax[i] = a0; ay[i] = a1; az[i] = a2;
-    if (pot) pot[i] = p;
+    double o[4];
+    direct_one(n, x, y, z, m, e2, i, o);
+    ax[i] = o[0]; ay[i] = o[1]; az[i] = o[2];
+    if (pot) pot[i] = o[3];
+  }
+}
+
+// The same sum for the targets tg[0..nt) only (bodies of all n act on each),
+// so the force error of a large system can be sampled without the O(n^2)
+// sum over every target.  Pinned by equality with orc_direct on those
+// targets (the same per-target loop).
+void orc_direct_subset(int64_t n, const double* x, const double* y, const double* z, const double* m,
+                       double eps, int64_t nt, const int64_t* tg, double* ax, double* ay, double* az, double* pot,
+                       int threads) {
+  const double e2 = eps * eps;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+  for (int64_t k = 0; k < nt; ++k) {
+    double o[4];
+    direct_one(n, x, y, z, m, e2, tg[k], o);
+    ax[k] = o[0]; ay[k] = o[1]; az[k] = o[2];
+    if (pot) pot[k] = o[3];
   }
 }
 

@@ -74,7 +74,7 @@ void dump(const std::string& dir, const char* name, const std::vector<T>& v) {
   std::fclose(f);
 }
 
-// Root set-up of the buildTree lambda (step.h:246-281), restated because the
+// Root set-up of the buildTree lambda (step.h:67-102), restated because the
 // lambda is local to Step.
 std::unique_ptr<Octree> make_root(const Particle& ps, double out[4]) {
   const int n = (int)ps.size();

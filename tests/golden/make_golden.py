@@ -109,6 +109,28 @@ def hdf5_case(name: str):
     print(f"{name}: hdf5")
 
 
+NEXT_REF = os.path.join(ROOT, "oracle", "_ref", "next_ref")
+
+
+def startup_case(name: str, threads: int = 4):
+    """The start-up lines NEXT's own driver prints (begrun.cpp:60-88): banner,
+    thread count, precision and particle count, for writers_small with
+    `threads` threads.  The driver loops until it reads 'q', so it runs in a
+    scratch directory for a moment and is stopped; only the lines before its
+    first dump line are kept."""
+    with tempfile.TemporaryDirectory() as td:
+        try:
+            r = subprocess.run([NEXT_REF, os.path.join(HERE, name + ".ic.txt"), str(threads), "0.001", "1000", "vtu"],
+                               cwd=td, stdin=subprocess.DEVNULL, capture_output=True, timeout=2)
+            out = r.stdout
+        except subprocess.TimeoutExpired as e:
+            out = e.stdout or b""
+    keep = out[: out.index(b"[Dump 0]")]
+    with open(os.path.join(HERE, f"{name}.startup{threads}.txt"), "wb") as f:
+        f.write(keep)
+    print(f"{name}: start-up lines ({len(keep)} bytes)")
+
+
 def hand_cases():
     """Small inputs that exercise single branches of the reference."""
     def mk(pos, m=None, types=None, vel=None):
@@ -160,10 +182,14 @@ def main():
     writer_case("writers_small", small)
     if os.path.exists(H5):
         hdf5_case("writers_small")
+    if os.path.exists(NEXT_REF):
+        startup_case("writers_small")
 
 
 if __name__ == "__main__":
     if sys.argv[1:] == ["hdf5"]:
         hdf5_case("writers_small")
+    elif sys.argv[1:] == ["startup"]:
+        startup_case("writers_small")
     else:
         main()

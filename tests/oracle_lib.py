@@ -48,6 +48,9 @@ def lib():
                                                        f64p]
         L.orc_direct.argtypes = [C.c_int64, f64p, f64p, f64p, f64p, C.c_double, f64p, f64p, f64p, f64p,
                                  C.c_int]
+        L.orc_direct_subset.argtypes = [C.c_int64, f64p, f64p, f64p, f64p, C.c_double, C.c_int64,
+                                        np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS"), f64p, f64p, f64p,
+                                        f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -147,4 +150,13 @@ def direct(x, y, z, m, eps, threads=8):
     out = [np.zeros(n) for _ in range(4)]
     lib().orc_direct(n, _c(x, np.float64), _c(y, np.float64), _c(z, np.float64), _c(m, np.float64), eps,
                      *out, threads)
+    return np.stack(out[:3], axis=1), out[3]
+
+
+def direct_subset(x, y, z, m, eps, targets, threads=8):
+    """FP64 direct sum on the given targets only (all bodies act on them)."""
+    tg = _c(targets, np.int64)
+    out = [np.zeros(len(tg)) for _ in range(4)]
+    lib().orc_direct_subset(len(x), _c(x, np.float64), _c(y, np.float64), _c(z, np.float64), _c(m, np.float64),
+                            eps, len(tg), tg, *out, threads)
     return np.stack(out[:3], axis=1), out[3]

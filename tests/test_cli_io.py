@@ -94,9 +94,13 @@ def test_writers_match_reference_bytes(tmp_path, fmt):
 
 
 def test_startup_lines(tmp_path):
+    """Banner (trailing spaces included), thread count, precision and
+    particle count, byte for byte as NEXT's own driver printed them
+    (begrun.cpp:60-88; fixture written by tests/golden/make_golden.py)."""
     r = run(os.path.join(GOLD, "writers_small.ic.txt"), 4, 0.001, 0.01, "vtu", "--dump-ic", tmp_path / "a.vtu")
-    lines = r.stdout.splitlines()
-    assert " Threads:   4" in lines and " Precision: FP64" in lines and " Particles: 5" in lines
+    with open(os.path.join(GOLD, "writers_small.startup4.txt"), "rb") as f:
+        want = f.read()
+    assert r.stdout.encode() == want
 
 
 def _count(tmp_path, text):

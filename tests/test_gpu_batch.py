@@ -55,15 +55,32 @@ def test_add_and_remove_mid_run(mode):
             e.run(1e-3, nsteps)
             return e.download()
 
+    def check_order(e, sizes):
+        """Device slots hold each system's bodies contiguously, in system
+        order, and their upload indices are a permutation of that system's."""
+        ids, sys_ = e.debug_order()
+        off = 0
+        for s, k in enumerate(sizes):
+            assert np.all(sys_[off:off + k] == s), s
+            assert np.array_equal(np.sort(ids[off:off + k]), np.arange(off, off + k, dtype=np.int32)), s
+            off += k
+        assert off == len(ids)
+
     with make(mode, True) as e:
         e.upload_batch([a, b])
+        check_order(e, [len(a), len(b)])
         e.run(1e-3, 3)
+        check_order(e, [len(a), len(b)])
         sc = e.add_system(c)
         assert sc == 2
+        check_order(e, [len(a), len(b), len(c)])
         e.run(1e-3, 2)
+        check_order(e, [len(a), len(b), len(c)])
         out_a = e.remove_system(0)           # a: 5 steps; b moves to index 0, c to 1
         assert e.num_systems() == 2
+        check_order(e, [len(b), len(c)])
         e.run(1e-3, 1)
+        check_order(e, [len(b), len(c)])
         out_b = e.download_system(0)         # b: 6 steps
         out_c = e.download_system(1)         # c: 3 steps
     assert np.array_equal(out_a, alone(a, 5))

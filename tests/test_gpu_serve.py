@@ -102,7 +102,7 @@ def test_idle_batches_release_memory_and_waits_are_safe():
         while srv.stats()["device_bytes"] > 1 << 20 and time.time() < deadline:
             time.sleep(0.01)
         st = srv.stats()
-        assert st["batches"] == 1 and st["running"] == 0 and st["bytes_in_use"] == 0
+        assert st["batches"] <= 1 and st["running"] == 0 and st["bytes_in_use"] == 0   # idle batches retire
         assert st["device_bytes"] <= 1 << 20         # was ~20,000 bodies' worth (> 10 MB)
 
 

@@ -181,3 +181,13 @@ def test_direct_sum_two_body_and_momentum():
     p = ic.plummer(500, seed=2)
     a, _ = orc.direct(p.x, p.y, p.z, p.m, 0.01)
     assert np.abs((p.m[:, None] * a).sum(0)).max() < 1e-12
+
+
+def test_direct_subset_equals_full_sum():
+    """The sampled direct sum (used for force errors of large systems) gives
+    the full sum's values on its targets, bit for bit."""
+    p = ic.plummer(700, seed=9)
+    a, pot = orc.direct(p.x, p.y, p.z, p.m, 0.02)
+    tg = np.array([0, 5, 699, 123, 5], np.int64)
+    b, pb = orc.direct_subset(p.x, p.y, p.z, p.m, 0.02, tg)
+    assert np.array_equal(b, a[tg]) and np.array_equal(pb, pot[tg])
