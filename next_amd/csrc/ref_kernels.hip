@@ -5,7 +5,7 @@
 // arithmetic operation by operation.
 //
 // Pipeline of one tree build (positions in `st`):
-//   bbox -> root cube (step.h:246-279)
+//   bbox -> root cube (step.h:68-102)
 //   keys: each body replays the FP64 node-centre recurrence of
 //         octree.h:49-64 for 21 levels -> 63-bit digit key
 //   stable radix sort of keys, gather of the state into key order
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(StateArrays st, const B
 }
 
 // Root centre and half-width of each system from its blocks' partials
-// (step.h:272-279), one wave per system; min and max are exact, so the
+// (step.h:93-100), one wave per system; min and max are exact, so the
 // order does not matter.
 __global__ __launch_bounds__(kWave) void k_bbox_final(const double* __restrict__ part,
                                                       const int32_t* __restrict__ block_off, SysArrays sy) {
@@ -460,39 +460,21 @@ __device__ __forceinline__ int64_t xcd_block(int nb) {
   return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
-// Correctly rounded sqrt(x) for normal x >= 2^-767: the v_rsq_f64 estimate
-// refined by Goldschmidt steps and two residual corrections.  This is the
-// compiler's own expansion of sqrt without its range scaling and special-
-// value selects, which are no-ops for the walk's arguments (r^2 + 1e-20 and
-// r^2 + eps^2 with eps >= 1e-4: positive, finite, far above 2^-767), so the
-// results are bit for bit those of sqrt.
-__device__ __forceinline__ double sqrt_rn(double x) {
+// Reciprocal square root and reciprocal with a relative error of a few
+// units in the last place: the v_rsq_f64 / v_rcp_f64 estimates refined by
+// one Newton step each (the residual formed with an FMA).  The walk's
+// arguments are positive, finite and mid-range: r^2 + 1e-20, r^2 + eps^2
+// with eps >= 1e-4, and 1 + 10 dist >= 1.
+__device__ __forceinline__ double rsq_nr(double x) {
   const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y;
-  double h = 0.5 * y;
-  const double r = __builtin_fma(-h, g, 0.5);
-  g = __builtin_fma(g, r, g);
-  h = __builtin_fma(h, r, h);
-  double d = __builtin_fma(-g, g, x);
-  g = __builtin_fma(d, h, g);
-  d = __builtin_fma(-g, g, x);
-  return __builtin_fma(d, h, g);
+  const double e = __builtin_fma(-(x * y), y, 1.0);        // 1 - x y^2
+  return __builtin_fma(0.5 * y, e, y);
 }
 
-// Correctly rounded 1/y for y in [2^-1000, 2^1000]: v_rcp_f64 refined by two
-// Newton steps, then the quotient residual correction.  The compiler's
-// division adds operand scaling (v_div_scale) and a special-value fixup
-// (v_div_fixup) that leave such operands unchanged, so the result is 1.0/y
-// bit for bit.  The walk's divisors are 1 + 10 dist >= 1 and
-// sqrt(r^2 + eps^2) >= 1e-4, both finite.
-__device__ __forceinline__ double rcp_rn(double y) {
-  double r = __builtin_amdgcn_rcp(y);
-  double e = __builtin_fma(-y, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-y, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-y, r, 1.0);
-  return __builtin_fma(e, r, r);
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, y, 1.0);              // 1 - x y
+  return __builtin_fma(y, e, y);
 }
 
 // Softening floor of a dark-matter target (octree.h:146-148), out of line:
@@ -594,37 +576,39 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
         const RefNode* nd = nodes + (e & 0x7fffffff);
         const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
         const double m = nd->m;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double dist = sqrt_rn(r2 + 1e-20);
-        double eps = nd->esum * rcp_rn(1.0 + dist * 10.0);   // softening.h:26-36
+        // octree.h:139-172 with products fused into FMAs and the square
+        // roots and divides faithfully rounded (a few ulp per interaction):
+        // r2 = |d|^2, dist = sqrt(r2 + 1e-20), eps from softening.h:26-36,
+        // dinv = 1/sqrt(r2 + eps^2).
+        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        const double r2e = r2 + 1e-20;
+        const double dist = r2e * rsq_nr(r2e);
+        double eps = nd->esum * rcp_nr(__builtin_fma(dist, 10.0, 1.0));
         if (eps < 1e-4) eps = 1e-4;
         if (pt == 1) eps = dm_softening(eps, nd->size, m, pm);   // octree.h:146-148
-        const double r2s = r2 + eps * eps;
-        const double dinv = rcp_rn(sqrt_rn(r2s));
-        const double inv3 = dinv * dinv * dinv;
-        const double fac = m * inv3;                        // G = 1: (1.0 * m) * inv3
-        a0 = a0 + dx * fac;
-        a1 = a1 + dy * fac;
-        a2 = a2 + dz * fac;
-        if (!leaf) {                                        // leaves carry Q = 0
+        const double dinv = rsq_nr(__builtin_fma(eps, eps, r2));
+        const double inv2 = dinv * dinv;
+        const double inv3 = inv2 * dinv;
+        double c = m * inv3;                                   // G = 1
+        if (!leaf) {                                           // leaves carry Q = 0
+          // quadrupole term 0.5 (2 (Q d) inv5 - 5 q inv7 d), q = d.Q.d
+          // (octree.h:159-172), as (Q d) inv5 + d (-(2.5 q) inv7)
           const double qxx = nd->qxx, qyy = nd->qyy, qzz = nd->qzz;
           const double qxy = nd->qxy, qxz = nd->qxz, qyz = nd->qyz;
-          const double inv5 = inv3 * (dinv * dinv);
-          const double inv7 = inv5 * (dinv * dinv);
-          const double q = qxx * dx * dx + qyy * dy * dy + qzz * dz * dz +
-                           2.0 * (qxy * dx * dy + qxz * dx * dz + qyz * dy * dz);
-          // 0.5 * (Qr * inv5 - 5 q inv7 d) with Qr = 2 (Q d) (octree.h:166-173),
-          // the powers of two moved inside: doubling and halving are exact,
-          // so (Q d) inv5 - ((2.5 q) inv7) d rounds to the same bits for every
-          // intermediate in the normal range (magnitudes above 2.2e-308).
-          const double w = 2.5 * q * inv7;
-          const double sx = qxx * dx + qxy * dy + qxz * dz;
-          const double sy = qxy * dx + qyy * dy + qyz * dz;
-          const double sz = qxz * dx + qyz * dy + qzz * dz;
-          a0 = a0 + (sx * inv5 - w * dx);
-          a1 = a1 + (sy * inv5 - w * dy);
-          a2 = a2 + (sz * inv5 - w * dz);
+          const double inv5 = inv3 * inv2;
+          const double inv7 = inv5 * inv2;
+          const double sx = __builtin_fma(qxz, dz, __builtin_fma(qxy, dy, qxx * dx));
+          const double sy = __builtin_fma(qyz, dz, __builtin_fma(qyy, dy, qxy * dx));
+          const double sz = __builtin_fma(qzz, dz, __builtin_fma(qyz, dy, qxz * dx));
+          const double q = __builtin_fma(dz, sz, __builtin_fma(dy, sy, dx * sx));
+          c = __builtin_fma(-2.5 * q, inv7, c);
+          a0 = __builtin_fma(sx, inv5, a0);
+          a1 = __builtin_fma(sy, inv5, a1);
+          a2 = __builtin_fma(sz, inv5, a2);
         }
+        a0 = __builtin_fma(dx, c, a0);
+        a1 = __builtin_fma(dy, c, a1);
+        a2 = __builtin_fma(dz, c, a2);
       }
     }
     head = (head + take) & (kWalkList - 1);
@@ -685,7 +669,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
   double v2 = 0.0;
   if (valid) {
     ax[i] = a0; ay[i] = a1; az[i] = a2;
-    if (KICK) {                                             // step.h:350-352
+    if (KICK) {                                             // step.h:171-173
       const double half = sy.dt[g.sys] * 0.5;
       const double vx = st.vx[i] + a0 * half;
       const double vy = st.vy[i] + a1 * half;
@@ -715,7 +699,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
   }
 }
 
-// Opening half kick and drift (step.h:294-325), each system with its dt.
+// Opening half kick and drift (step.h:120-122, 141-146), each system with its dt.
 // amag (may be null) receives |a| of each body, in FP32, for an opening
 // criterion relative to the previous acceleration.
 __global__ __launch_bounds__(kBlock) void k_kick_drift(int64_t n, StateArrays st, const double* __restrict__ ax,
