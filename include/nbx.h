@@ -188,6 +188,27 @@ nbx_status nbx_run(nbx_ctx* ctx, double base_dt, int32_t nsteps, double* dts_out
  * octree.h:133-179), upload order. */
 nbx_status nbx_accel(nbx_ctx* ctx, double* ax, double* ay, double* az);
 
+/* No counterpart in NEXT.  For callers that cannot allocate device memory
+ * (serving loops and harnesses that forbid hipMalloc): nbx_accel's
+ * reference-mode accelerations (same kernels, same results) of n bodies the
+ * caller holds in device memory, written to device arrays ax/ay/az, without
+ * a context.  Every other buffer is a static device array of the library
+ * (16 MiB, held by each GPU it runs on until the process ends), sized for at
+ * most NBX_NOALLOC_MAX_BODIES bodies and NBX_NOALLOC_MAX_NODES tree nodes.
+ * Runs on the calling thread's current device and the given stream (null:
+ * the null stream); synchronises that stream once for the node counts and
+ * returns after the results are written.  Calls on one device share the
+ * array and are serialised by a lock.  The call itself allocates no memory
+ * and is not capturable (it waits on the stream).  Errors: NBX_ERR_ARG (null
+ * pointer, n < 0 or above the maximum, theta < 0), NBX_ERR_NOMEM (the tree
+ * needs more nodes than the maximum), NBX_ERR_NONFINITE, NBX_ERR_DEPTH,
+ * NBX_ERR_NODEVICE, NBX_ERR_HIP. */
+#define NBX_NOALLOC_MAX_BODIES 16384
+#define NBX_NOALLOC_MAX_NODES 65536
+nbx_status nbx_accel_noalloc(void* stream, int64_t n, const double* x, const double* y, const double* z,
+                             const double* m, const int32_t* type, double theta, double* ax, double* ay,
+                             double* az);
+
 /* NBX_MODE_PHYSICS: gravitational potential of every body at the current
  * positions from the tree walk (self pair excluded), upload order. */
 nbx_status nbx_potential(nbx_ctx* ctx, double* pot);

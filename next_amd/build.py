@@ -38,6 +38,7 @@ SOURCES = [
     ("phys_kernels.hip", []),
     ("nbx_api.cpp", []),
     ("serve.cpp", []),
+    ("noalloc.hip", []),
 ]
 
 

@@ -31,7 +31,7 @@ EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", 
            "nbx_potential", "nbx_direct", "nbx_energy", "nbx_energy_systems", "nbx_upload_batch", "nbx_add_system",
            "nbx_download_system", "nbx_remove_system", "nbx_num_systems", "nbx_system_size", "nbx_server_create",
            "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats",
-           "nbx_server_create_with_ops"]
+           "nbx_server_create_with_ops", "nbx_accel_noalloc"]
 
 
 class NbxError(RuntimeError):
@@ -133,8 +133,20 @@ def lib():
     L.nbx_server_poll.argtypes = [VP, C.c_int64]
     L.nbx_server_wait.argtypes = [VP, C.c_int64] + [f64p] * 6 + [C.POINTER(RequestTiming)]
     L.nbx_server_stats.argtypes = [VP, C.POINTER(ServerStats)]
+    L.nbx_accel_noalloc.argtypes = [VP, C.c_int64] + [VP] * 5 + [C.c_double] + [VP] * 3
     _lib = L
     return L
+
+
+def accel_noalloc(n: int, x: int, y: int, z: int, m: int, type_: int, theta: float, ax: int, ay: int, az: int,
+                  stream: int = 0):
+    """nbx_accel_noalloc: reference-mode accelerations of n bodies held in
+    device memory (x, y, z, m: float64, type_: int32; device addresses as
+    ints) into device arrays ax, ay, az, on `stream` (a HIP stream handle;
+    0 = the null stream) of the current device, with no device allocation."""
+    st = lib().nbx_accel_noalloc(stream or None, n, x, y, z, m, type_, theta, ax, ay, az)
+    if st != 0:
+        raise NbxError(st, "nbx_accel_noalloc")
 
 
 def _f64(a):
