@@ -12,15 +12,12 @@ constexpr int kKeyDigits = 21;     // octree digits per 64-bit key word
 constexpr int kMaxLevels = 42;     // two key words
 constexpr int kMaxDepthSlots = 64; // histogram slots for node depths
 
-// One tree node in depth-first preorder, laid out for the walk: a wave reads
-// one node at a time at a wave-uniform index, so the record is fetched
-// through the scalar cache and broadcast to all lanes.  The fields every
-// visit needs sit in the first 64 bytes; the quadrupole, needed only when a
-// cell is accepted, in the second.  128 bytes = one L2 line.
+// One tree node in depth-first preorder, children in digit order: what the
+// upward pass and the walk's evaluation read (the traversal reads the
+// node's WalkHead below).  128 bytes = one L2 line.
+constexpr int32_t kLinkZeroMass = 1 << 30;   // link flag: zero mass, skipped whole
 struct alignas(128) RefNode {
-  // first 56 bytes: what the walk reads for every node it visits
   double cx, cy, cz, m;    // centre of mass, mass
-  double acc2, open2;      // (size/theta)^2 (1 +- 1e-12): accept beyond, open below
   int32_t skip;            // preorder index of the first node after this subtree
   int32_t body;            // leaf: sorted body index; internal: -1
   int32_t depth;
@@ -28,8 +25,23 @@ struct alignas(128) RefNode {
   double size;             // half-width: root half-width * 2^-depth
   double esum;             // size*0.015 + cbrt(m)*0.002 (softening numerator)
   double qxx, qyy, qzz, qxy, qxz, qyz;
+  double pad_[2];
 };
 static_assert(sizeof(RefNode) == 128, "RefNode layout");
+
+// The per-visit fields of a node for the walk's traversal, in an array of
+// their own beside the node records (two per 64-byte scalar cache line): a
+// visited node costs one 32-byte scalar load.  acc2 is the upper edge of the
+// opening test's band in FP32, (size/theta)^2 (1 + 2e-6) rounded up, or NaN
+// where that is out of FP32's normal range (every decision then exact).
+struct alignas(32) WalkHead {
+  double cx, cy, cz;       // centre of mass
+  float acc2;
+  // leaf: ~body (negative); internal: skip; with kLinkZeroMass set (and the
+  // skip target) when the node's mass is zero
+  int32_t link;
+};
+static_assert(sizeof(WalkHead) == 32, "WalkHead layout");
 
 // Small per-context device block: flags and the per-level node counts the
 // host needs to launch the upward pass.

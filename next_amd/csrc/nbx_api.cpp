@@ -57,7 +57,7 @@ struct nbx_ctx {
   DBuf acc[3];
   DBuf key[2], idx[2], key2[2];
   DBuf lcp, cnt_s, cnt_e;
-  DBuf nodes;
+  DBuf nodes, heads;          // reference-mode node records and their walk heads
   DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
   DBuf scratch, dts, lvl_part;
   DBuf pnodes, paux, pot, epart, eout;     // physics-mode tree, moments, potential, energy
@@ -165,7 +165,7 @@ void each_buf(nbx_ctx* c, F f) {
   for (auto& w : c->st) for (auto& b : w) f(b);
   for (DBuf* a : {c->acc, c->key, c->idx, c->key2, c->amag})
     for (int k = 0; k < (a == c->acc ? 3 : 2); ++k) f(a[k]);
-  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
+  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->heads, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
                   &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
                   &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->dacc, &c->epart, &c->eout, &c->s_root, &c->s_dt,
                   &c->s_v2, &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part,
@@ -344,12 +344,12 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
                      P<uint32_t>(c->en_val), P<int32_t>(c->gmarks), P<uint32_t>(c->gmarks) + n, c->spec_nodes,
                      c->spec_int, s);
     c->spec_emit = true;
-  } else if (!c->physics && c->nodes.p && ci > 0) {
-    c->spec_nodes = cap_words(c->nodes, sizeof(RefNode));
+  } else if (!c->physics && c->nodes.p && c->heads.p && ci > 0) {
+    c->spec_nodes = std::min(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead)));
     c->spec_int = ci;
     launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, sy, c->sc,
-                P<RefNode>(c->nodes), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
-                P<uint32_t>(c->en_val), c->spec_nodes, c->spec_int, s);
+                P<RefNode>(c->nodes), P<WalkHead>(c->heads), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val),
+                P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), c->spec_nodes, c->spec_int, s);
     c->spec_emit = true;
   }
   *skey_out = skey;
@@ -367,12 +367,14 @@ nbx_status ref_layout(nbx_ctx* c, bool emitted) {
   // The walk reads node records at 32-bit byte offsets, and one record past
   // the last node (a successor it loads but never uses).
   NBXCHK(ensure(c, c->nodes, (size_t)(total + 1) * sizeof(RefNode)));
+  NBXCHK(ensure(c, c->heads, (size_t)(total + 1) * sizeof(WalkHead)));
   uint32_t* scr = P<uint32_t>(c->scratch);
   RefNode* nodes = P<RefNode>(c->nodes);
   if (!emitted)
     launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), sysa(c),
-                c->sc, nodes, P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
-                P<uint32_t>(c->en_val), cap_words(c->nodes, sizeof(RefNode)), cap_int(c), s);
+                c->sc, nodes, P<WalkHead>(c->heads), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val),
+                P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
+                std::min(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead))), cap_int(c), s);
   const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
                                  P<uint32_t>(c->st_val2), m, 8, scr, s);
   const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
@@ -511,7 +513,8 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     RefNode* nodes = P<RefNode>(c->nodes);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
-        launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, c->cfg.theta, s);
+        launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<WalkHead>(c->heads),
+                      c->cfg.theta, s);
   }
   record(c, EV_UPWARD);
   c->tree_valid = true;
@@ -570,7 +573,8 @@ nbx_status walk(nbx_ctx* c, bool kick) {
                      c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64,
                      c->amag_live ? P<float>(c->amag[c->acur]) : nullptr, c->alpha, c->cfg.theta, c->stream);
   else
-    launch_walk(P<Group>(c->groups), c->ngroups, P<RefNode>(c->nodes), view(c, c->cur), sysa(c), c->cfg.theta,
+    launch_walk(P<Group>(c->groups), c->ngroups, P<RefNode>(c->nodes), P<WalkHead>(c->heads), view(c, c->cur),
+                sysa(c), c->cfg.theta,
                 P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), c->sc, kick, count, c->stream);
   record(c, EV_WALK);
   HIPCHK(c, hipGetLastError());
@@ -1299,7 +1303,9 @@ nbx_status nbx_debug_tree(nbx_ctx* c, int32_t* depth, int32_t* leaf, int32_t* bo
     for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
       const int64_t cnt = c->hsc->level_count[d];
       off -= cnt;
-      if (cnt > 0) launch_upward(P<int32_t>(c->level_list), off, cnt, P<RefNode>(c->nodes), c->cfg.theta, c->stream);
+      if (cnt > 0)
+        launch_upward(P<int32_t>(c->level_list), off, cnt, P<RefNode>(c->nodes), P<WalkHead>(c->heads), c->cfg.theta,
+                      c->stream);
     }
   }
   // one staging allocation for all outputs

@@ -39,7 +39,7 @@ constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // Byte offsets of every buffer inside the array, for kNaBodies bodies and
 // kNaNodes nodes.
 struct Layout {
-  size_t sc, st[2][10], acc[3], key[2], idx[2], key2[2], lcp, cnt_s, cnt_e, part, sortscr, nodes, ik[9];
+  size_t sc, st[2][10], acc[3], key[2], idx[2], key2[2], lcp, cnt_s, cnt_e, part, sortscr, nodes, heads, ik[9];
   size_t root, dt, v2, soff, sroot, send, groups, boxes, box_off, box_part, total;
 };
 
@@ -65,6 +65,7 @@ constexpr Layout make_layout() {
   const size_t cw = (size_t)((big + kScanTile - 1) / kScanTile + 2);
   L.sortscr = take((sw > cw ? sw : cw) * 4);
   L.nodes = take((size_t)(kNaNodes + 1) * sizeof(RefNode));   // one spare record for the walk
+  L.heads = take((size_t)(kNaNodes + 1) * sizeof(WalkHead));
   for (int k = 0; k < 9; ++k) L.ik[k] = take((size_t)kNaNodes * 4);
   L.root = take(4 * 8);
   L.dt = take(8);
@@ -246,7 +247,8 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   uint32_t* ik[9];
   for (int k = 0; k < 9; ++k) ik[k] = A.at<uint32_t>(kLayout.ik[k]);
   // ik: st_key st_val en_key en_val st_key2 st_val2 en_key2 en_val2 level_list
-  launch_emit(n, lcp, cnt_s, cnt_e, St, sy, sc, nodes, ik[0], ik[1], ik[2], ik[3], (uint32_t)(kNaNodes + 1),
+  WalkHead* heads = A.at<WalkHead>(kLayout.heads);
+  launch_emit(n, lcp, cnt_s, cnt_e, St, sy, sc, nodes, heads, ik[0], ik[1], ik[2], ik[3], (uint32_t)(kNaNodes + 1),
               (uint32_t)kNaNodes, s);
   const bool sa = radix_sort_u32(ik[0], ik[1], ik[4], ik[5], mi, 8, scr, s);
   const bool ea = radix_sort_u32(ik[2], ik[3], ik[6], ik[7], mi, 8, scr, s);
@@ -256,11 +258,11 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
     const int64_t cnt = h.level_count[d];
     off -= cnt;
-    if (cnt > 0) launch_upward(level_list, off, cnt, nodes, theta, s);
+    if (cnt > 0) launch_upward(level_list, off, cnt, nodes, heads, theta, s);
   }
   double* acc[3] = {A.at<double>(kLayout.acc[0]), A.at<double>(kLayout.acc[1]), A.at<double>(kLayout.acc[2])};
-  launch_walk(A.at<Group>(kLayout.groups), div_up(n, kWave), nodes, St, sy, theta, acc[0], acc[1], acc[2], sc, false,
-              false, s);
+  launch_walk(A.at<Group>(kLayout.groups), div_up(n, kWave), nodes, heads, St, sy, theta, acc[0], acc[1], acc[2], sc,
+              false, false, s);
   hipLaunchKernelGGL(k_na_scatter, dim3((unsigned)div_up(n, kBlock)), dim3(kBlock), 0, s, n, St.id, acc[0], acc[1],
                      acc[2], ax, ay, az);
   NA_HIP(hipGetLastError());

@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
                                                  const uint32_t* __restrict__ pre_off,
                                                  const uint32_t* __restrict__ end_off, StateArrays st,
                                                  SysArrays sy, const DevScalars* __restrict__ sc,
-                                                 RefNode* __restrict__ nodes,
+                                                 RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
                                                  uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
                                                  uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val,
                                                  uint32_t cap_nodes, uint32_t cap_int) {
@@ -368,6 +368,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
     nd.body = (int32_t)i;
     nd.depth = d;
     nd.first = (int32_t)i;
+    // a leaf is accepted whatever its distance: no threshold
+    heads[p] = WalkHead{nd.cx, nd.cy, nd.cz, 0.0f, m == 0.0 ? (int32_t)(p + 1) | kLinkZeroMass : ~(int32_t)i};
   }
   const int ne = lp > li ? lp - li : 0;
   if (ne) {
@@ -395,7 +397,8 @@ __global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const uint32_t* __re
 // nodes of one depth from their children (octree.h:104-126), and the
 // squared-distance thresholds of the walk's opening test for this theta.
 __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt,
-                                                   RefNode* __restrict__ nodes, double inv_theta) {
+                                                   RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
+                                                   double inv_theta) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= cnt) return;
   const int32_t id = level_list[beg + k];
@@ -448,8 +451,8 @@ __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ l
   p.qxx = qxx; p.qyy = qyy; p.qzz = qzz; p.qxy = qxy; p.qxz = qxz; p.qyz = qyz;
   const double q = size * inv_theta;
   const double q2 = q * q;
-  p.acc2 = q2 * (1.0 + 1e-12);
-  p.open2 = q2 * (1.0 - 1e-12);
+  const float acc2 = (q2 > 1e-36 && q2 < 1e36) ? __double2float_ru(q2 * (1.0 + 2e-6)) : __builtin_nanf("");
+  heads[id] = WalkHead{cx, cy, cz, acc2, M == 0.0 ? (end | kLinkZeroMass) : end};
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks
@@ -485,37 +488,59 @@ __device__ __noinline__ double dm_softening(double eps, double size, double m, d
   return eps < alt ? alt : eps;
 }
 
-// The per-visit fields of a node, held in scalar registers.
-struct NodeHead {
-  double cx, cy, cz, m, acc2, open2;
-  int32_t skip, body;
-};
-
 // k < 2^25, checked on the host: the byte offset fits 32 bits and becomes the
 // scalar load's offset register.
-__device__ __forceinline__ NodeHead load_head(const RefNode* __restrict__ nodes, uint32_t k) {
-  const RefNode* p = reinterpret_cast<const RefNode*>(reinterpret_cast<const char*>(nodes) + (k << 7));
-  NodeHead h;
-  h.cx = p->cx; h.cy = p->cy; h.cz = p->cz; h.m = p->m;
-  h.acc2 = p->acc2; h.open2 = p->open2; h.skip = p->skip; h.body = p->body;
+// The per-visit fields of a node as the traversal holds them: the head,
+// loaded as one 8-dword value (one scalar load), with the FP32 threshold
+// kept as its bits until it is used.  (Read as a float straight from the
+// loaded vector, the threshold was folded to element 0 instead of 6 by the
+// ROCm 7.2 compiler, which wrecked every opening decision; the integer
+// extraction compiles correctly.)
+typedef uint32_t u32x8 __attribute__((ext_vector_type(8)));
+struct HeadRegs {
+  double cx, cy, cz;
+  uint32_t acc2_bits;
+  int32_t link;
+};
+__device__ __forceinline__ HeadRegs load_head(const WalkHead* __restrict__ heads, uint32_t k) {
+  const u32x8 r = *reinterpret_cast<const u32x8*>(reinterpret_cast<const char*>(heads) + (k << 5));
+  HeadRegs h;
+  h.cx = __builtin_bit_cast(double, (uint64_t)r[0] | ((uint64_t)r[1] << 32));
+  h.cy = __builtin_bit_cast(double, (uint64_t)r[2] | ((uint64_t)r[3] << 32));
+  h.cz = __builtin_bit_cast(double, (uint64_t)r[4] | ((uint64_t)r[5] << 32));
+  h.acc2_bits = r[6];
+  h.link = (int32_t)r[7];
   return h;
 }
 
-// Accepted nodes a lane holds (a ring in LDS, a power of two), and how many
-// of each lane's oldest entries one evaluation round takes.  Evaluating only
-// the oldest entries when some lane's ring is full keeps more lanes busy per
+// Lower edge of the opening test's band from its upper edge acc2 (FP32,
+// (size/theta)^2 (1 + 2e-6) rounded up): below (size/theta)^2 (1 - 1e-6).
+// Squared distances between the edges are decided by NEXT's expression.
+constexpr double kBandLow = (1.0 - 2e-6) / (1.0 + 2.2e-6);
+
+// Accepted nodes a lane holds (a ring in LDS), and how many of each lane's
+// oldest entries one evaluation round takes.  Evaluating only the oldest
+// entries when some lane's ring is full keeps more lanes busy per
 // evaluation step than draining every ring: lanes fill at different rates.
-// 32 entries (8 KB of LDS per wave: 5 waves per SIMD, where the registers
-// would allow 6) measured 3 % faster than 16, the evaluation steps being 69 %
-// busy instead of 54 %; 64 entries (2.5 waves per SIMD) were 30 % slower.
-constexpr int kWalkList = 32;
+// The walk is bound by the latency of its dependent node loads, so resident
+// waves count: 24 entries (6 KB of LDS per wave) with the registers held to
+// 80 (a few spills outside the loops) give 6 waves per SIMD, 6 % faster
+// than 32 entries at 5; 20 entries at 7 waves (17 spills) were slower, and so
+// were 16 (fewer lanes busy per evaluation step) and 64 (2.5 waves).
+constexpr int kWalkList = 24;
 // One wave per workgroup: a wave's registers and LDS are released as soon as
 // its group is done instead of when the slowest wave of a larger workgroup
 // is (measured 2.39 -> 2.34 ms against four waves per workgroup).
 constexpr int kWalkBlock = 64;
 constexpr int kWalkFlush = 4;
-static_assert((kWalkList & (kWalkList - 1)) == 0 && kWalkFlush <= kWalkList, "walk ring");
+static_assert(kWalkFlush <= kWalkList, "walk ring");
 
+// Node records come in through the scalar cache: the traversal loads one
+// 32-byte WalkHead per visited node, the evaluation a lane's accepted
+// records by vector loads.  Loading the next entry's record while the
+// current one is evaluated measured slower (it costs the registers of a
+// sixth wave).
+//
 // Two phases per wave.  Traversal: every lane follows its own depth-first
 // path (open: go to the first child, cur+1; accept or skip: go past the
 // subtree, skip[cur]).  The wave visits node `cur` and the lanes whose own
@@ -533,13 +558,16 @@ static_assert((kWalkList & (kWalkList - 1)) == 0 && kWalkFlush <= kWalkList, "wa
 // octree.h:158-177, so the sums are NEXT's bit for bit.
 //
 // Opening test of octree.h:153, (size/dist) < theta with
-// dist = sqrt(r2 + 1e-20): decided from r2 + 1e-20 against (size/theta)^2
-// when it lies outside a relative band of 1e-12 (rounding errors of either
-// form are below 1e-15), and by the exact expression inside the band, so the
-// decision is NEXT's bit for bit and opened nodes cost no sqrt or divide.
+// dist = sqrt(r2 + 1e-20): decided from r2 + 1e-20 against the node's FP32
+// band edges around (size/theta)^2 (relative width about 4e-6; rounding
+// errors of either form are below 1e-15), and by NEXT's exact expression
+// inside the band, so the decision is NEXT's bit for bit and opened nodes
+// cost no sqrt or divide.
 template <bool KICK, bool COUNT>
-__global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ groups, int64_t ngroups,
-                                                 const RefNode* __restrict__ nodes, StateArrays st, SysArrays sy,
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(6)))
+void k_walk(const Group* __restrict__ groups, int64_t ngroups,
+                                                 const RefNode* __restrict__ nodes,
+                                                 const WalkHead* __restrict__ heads, StateArrays st, SysArrays sy,
                                                  double theta, double* __restrict__ ax, double* __restrict__ ay,
                                                  double* __restrict__ az, DevScalars* sc) {
   __shared__ int32_t lists[kWalkBlock / kWave][kWalkList][kWave];
@@ -558,7 +586,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
   double px = 0, py = 0, pz = 0, pm = 1;
   int pt = 0;
   if (valid) { px = st.x[i]; py = st.y[i]; pz = st.z[i]; pm = st.m[i]; pt = st.type[i]; }
-  const int32_t me = valid ? (int32_t)i : -2;
+  const int32_t self_link = valid ? ~(int32_t)i : INT_MAX;   // link of this lane's own leaf
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   uint32_t npp = 0, npc = 0, nop = 0, nit = 0, nev = 0;
   int head = 0, len = 0;                                   // this lane's ring: [head, head + len)
@@ -568,10 +596,12 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
   // (Q = 0).
   auto evaluate = [&](int limit) {
     const int take = len < limit ? len : limit;
-    for (int k = 0; __ballot(k < take); ++k) {
+    for (int k = 0; __builtin_amdgcn_ballot_w64(k < take); ++k) {
       if (COUNT) ++nev;
       if (k < take) {
-        const int32_t e = mine[((head + k) & (kWalkList - 1)) * kWave];
+        int slot = head + k;
+        if (slot >= kWalkList) slot -= kWalkList;
+        const int32_t e = mine[slot * kWave];
         const bool leaf = e < 0;
         const RefNode* nd = nodes + (e & 0x7fffffff);
         const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
@@ -611,14 +641,15 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
         a2 = __builtin_fma(dz, c, a2);
       }
     }
-    head = (head + take) & (kWalkList - 1);
+    head += take;
+    if (head >= kWalkList) head -= kWalkList;
     len -= take;
   };
 
   const int32_t root = sy.sroot[g.sys];
   int32_t next = valid ? root : INT_MAX;
   int32_t cur = root;
-  NodeHead h = load_head(nodes, root);
+  HeadRegs h = load_head(heads, root);
   // Traversal runs until some lane's ring is full (or the tree is done),
   // then one evaluation round; the running sums are not touched inside the
   // traversal loop, which keeps them out of its loop-carried registers.
@@ -627,39 +658,43 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(const Group* __restrict__ g
       // successors, loaded while this node is processed
       // (the node array has one spare record after the last node, so both
       // successors, at most total, are readable)
-      const int32_t sk = h.skip > cur ? h.skip : cur + 1;   // skip > cur by construction
-      const NodeHead h1 = load_head(nodes, (uint32_t)cur + 1u);
-      const NodeHead h2 = load_head(nodes, (uint32_t)sk);
-      bool open = false;
-      if (next == cur) {
-        if (h.m == 0.0 || h.body == me) {                    // octree.h:134-135
-          next = sk;
-        } else {
-          const bool leaf = h.body >= 0;
-          bool accept = leaf;
-          if (!leaf) {
-            const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
-            const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
-            if (r2e > h.acc2) accept = true;
-            else if (r2e < h.open2) accept = false;
-            else accept = (nodes[cur].size / sqrt(r2e)) < theta;
-          }
-          if (accept) {
-            mine[((head + len) & (kWalkList - 1)) * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
-            ++len;
-            next = sk;
-            if (COUNT) { if (leaf) ++npp; else ++npc; }
-          } else {
-            open = true;                                      // first child
-            next = cur + 1;
-            if (COUNT) ++nop;
-          }
-        }
+      const bool leaf = h.link < 0;
+      const int32_t lk = leaf ? cur + 1 : (h.link & ~kLinkZeroMass);
+      const int32_t sk = lk > cur ? lk : cur + 1;           // skip > cur by construction
+      const HeadRegs h1 = load_head(heads, (uint32_t)cur + 1u);
+      const HeadRegs h2 = load_head(heads, (uint32_t)sk);
+      // This lane decides cur if its own path is at cur.  The decision is
+      // formed without branching on it: every lane computes the test, and
+      // every lane writes cur into its ring slot head+len, which is free
+      // (each ring holds fewer than kWalkList entries inside this loop) and
+      // becomes an entry only if the lane accepts.
+      const bool here = next == cur;
+      const bool pass = (h.link >= 0 && (h.link & kLinkZeroMass)) || h.link == self_link;   // octree.h:134-135
+      bool accept = leaf;
+      if (!leaf) {                                            // wave-uniform
+        const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
+        const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
+        const double hi = (double)__uint_as_float(h.acc2_bits);   // NaN: no fast decision
+        accept = r2e > hi;
+        const bool band = here && !accept && !(r2e < hi * kBandLow);
+        if (band) accept = (nodes[cur].size / sqrt(r2e)) < theta;   // rare: NEXT's exact expression
+      }
+      const bool take = here && !pass && accept;
+      const bool open = here && !pass && !accept;
+      int slot = head + len;
+      if (slot >= kWalkList) slot -= kWalkList;
+      mine[slot * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
+      len += take ? 1 : 0;
+      next = here ? (open ? cur + 1 : sk) : next;
+      if (COUNT) {
+        npp += (take && leaf) ? 1u : 0u;
+        npc += (take && !leaf) ? 1u : 0u;
+        nop += open ? 1u : 0u;
       }
       if (COUNT) ++nit;
-      if (__ballot(open)) { cur = cur + 1; h = h1; }
+      if (__builtin_amdgcn_ballot_w64(open)) { cur = cur + 1; h = h1; }
       else { cur = sk; h = h2; }
-      if (__ballot(len == kWalkList)) break;
+      if (__builtin_amdgcn_ballot_w64(len == kWalkList)) break;
     }
     if (cur >= total) break;
     evaluate(kWalkFlush);
@@ -885,31 +920,33 @@ void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevSca
   hipLaunchKernelGGL(k_sys_roots, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, pre_off, sc);
 }
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
-                 const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
-                 uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes, uint32_t cap_int,
-                 hipStream_t s) {
+                 const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, WalkHead* heads,
+                 uint32_t* st_key, uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes,
+                 uint32_t cap_int, hipStream_t s) {
   if (n > 0)
     hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sy, sc, nodes,
-                       st_key, st_val, en_key, en_val, cap_nodes, cap_int);
+                       heads, st_key, st_val, en_key, en_val, cap_nodes, cap_int);
 }
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
                  hipStream_t s) {
   if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, st_val, en_val, nodes, level_list);
 }
-void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, double theta,
-                   hipStream_t s) {
+void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
+                   double theta, hipStream_t s) {
   if (cnt > 0)
-    hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes, 1.0 / theta);
+    hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes, heads,
+                       1.0 / theta);
 }
-void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
+void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const WalkHead* heads,
+                 const StateArrays& st,
                  const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
                  bool count, hipStream_t s) {
   if (ngroups <= 0) return;
   const int wpb = kWalkBlock / kWave;
   const unsigned nb = (unsigned)(div_up(div_up(ngroups, wpb), 8) * 8);
 #define NBX_RW(K, C) \
-  hipLaunchKernelGGL((k_walk<K, C>), dim3(nb), dim3(kWalkBlock), 0, s, groups, ngroups, nodes, st, sy, theta, ax, ay, \
-                     az, sc)
+  hipLaunchKernelGGL((k_walk<K, C>), dim3(nb), dim3(kWalkBlock), 0, s, groups, ngroups, nodes, heads, st, sy, theta, \
+                     ax, ay, az, sc)
   if (kick && count) NBX_RW(true, true);
   else if (kick) NBX_RW(true, false);
   else if (count) NBX_RW(false, true);

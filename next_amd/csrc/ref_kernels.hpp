@@ -45,20 +45,21 @@ void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevSca
 void launch_remove_map(int64_t n_new, int64_t cut, int64_t len, uint32_t* perm, hipStream_t s);
 void launch_relabel(int64_t n, int32_t* sys, int32_t removed, hipStream_t s);
 void launch_gather_f64(int64_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s);
-// Node records and depth-sort keys from the sorted keys' lcp and the two
-// scans; writes only node records below cap_nodes and key entries below
+// Node records (and the walk heads of the leaves) and depth-sort keys from
+// the sorted keys' lcp and the two scans; writes only records below cap_nodes and key entries below
 // cap_int (the buffers' capacities), so it can run before the host knows the
 // node counts.
 void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
-                 const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, uint32_t* st_key,
-                 uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes, uint32_t cap_int,
-                 hipStream_t s);
+                 const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, WalkHead* heads,
+                 uint32_t* st_key, uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes,
+                 uint32_t cap_int, hipStream_t s);
 void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
                  hipStream_t s);
-void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, double theta,
-                   hipStream_t s);
+void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
+                   double theta, hipStream_t s);
 void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s);
-void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const StateArrays& st,
+void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const WalkHead* heads,
+                 const StateArrays& st,
                  const SysArrays& sy, double theta, double* ax, double* ay, double* az, DevScalars* sc, bool kick,
                  bool count, hipStream_t s);
 // amag (may be null): |a| per body in FP32.
