@@ -49,8 +49,7 @@ struct DevScalars {
   int32_t err;              // bit 0: non-finite position
   int32_t collisions;       // adjacent equal primary keys
   int32_t coincident;       // adjacent equal primary and secondary keys
-  int32_t total_nodes;
-  int32_t internal_nodes;
+  int32_t total_nodes;              // (internal nodes: the sum of level_count)
   int32_t max_leaf_depth;
   int32_t level_count[kMaxDepthSlots];  // internal nodes per depth
   unsigned long long n_pp, n_pc;        // interaction counters

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "layout.hpp"
 #include "nbx.h"
 #include "phys_kernels.hpp"
 #include "prims.hpp"
@@ -56,9 +57,9 @@ struct nbx_ctx {
   int cur = 0;
   DBuf acc[3];
   DBuf key[2], idx[2], key2[2];
-  DBuf lcp, cnt_s, cnt_e;
+  DBuf lcp, cnt_s;
   DBuf nodes, heads;          // reference-mode node records and their walk heads
-  DBuf st_key, st_val, en_key, en_val, st_key2, st_val2, en_key2, en_val2, level_list;
+  DBuf level_list, end_list;  // internal nodes by depth, and their skip targets
   DBuf scratch, dts, lvl_part;
   DBuf pnodes, paux, pot, epart, eout;     // physics-mode tree, moments, potential, energy
   DBuf dacc;                  // forces of the diagnostic passes (potential, direct energy)
@@ -165,8 +166,7 @@ void each_buf(nbx_ctx* c, F f) {
   for (auto& w : c->st) for (auto& b : w) f(b);
   for (DBuf* a : {c->acc, c->key, c->idx, c->key2, c->amag})
     for (int k = 0; k < (a == c->acc ? 3 : 2); ++k) f(a[k]);
-  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->cnt_e, &c->nodes, &c->heads, &c->st_key, &c->st_val, &c->en_key, &c->en_val,
-                  &c->st_key2, &c->st_val2, &c->en_key2, &c->en_val2, &c->level_list, &c->scratch, &c->dts,
+  for (DBuf* b : {&c->lcp, &c->cnt_s, &c->nodes, &c->heads, &c->level_list, &c->end_list, &c->scratch, &c->dts,
                   &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->dacc, &c->epart, &c->eout, &c->s_root, &c->s_dt,
                   &c->s_v2, &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part,
                   &c->tgroups, &c->gmarks})
@@ -179,8 +179,7 @@ uint32_t cap_words(const DBuf& b, size_t esz) { return (uint32_t)std::min<size_t
 
 // Entries the four depth-sort key arrays can hold.
 uint32_t cap_int(const nbx_ctx* c) {
-  return std::min({cap_words(c->st_key, 4), cap_words(c->st_val, 4), cap_words(c->en_key, 4),
-                   cap_words(c->en_val, 4)});
+  return std::min(cap_words(c->level_list, 4), cap_words(c->end_list, 4));
 }
 
 StateArrays view(nbx_ctx* c, int which) {
@@ -220,7 +219,6 @@ nbx_status ensure_particles(nbx_ctx* c, int64_t n, int64_t keep_n) {
     }
     NBXCHK(ensure(c, c->lcp, (size_t)cap * 4));
     NBXCHK(ensure(c, c->cnt_s, (size_t)cap * 4));
-    NBXCHK(ensure(c, c->cnt_e, (size_t)cap * 4));
     NBXCHK(ensure(c, c->lvl_part, (size_t)lcp_part_words(cap) * 4));
     NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(cap), scan_scratch_words(cap)) * 4));
     c->cap = cap;
@@ -322,11 +320,9 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   record(c, EV_SORT);
 
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
-  uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
-  launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
-                   P<uint32_t>(c->lvl_part), c->sc, s);
+  launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
+                   c->sc, s);
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
-  scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
   HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipEventRecord(c->ev_scalars, s));
   // Lay out the nodes now, into the buffers of the last build, while the
@@ -339,17 +335,17 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
     launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
     c->spec_nodes = cap_words(c->pnodes, sizeof(PhysNode));
     c->spec_int = ci;
-    launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, c->sc,
-                     P<PhysNode>(c->pnodes), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
-                     P<uint32_t>(c->en_val), P<int32_t>(c->gmarks), P<uint32_t>(c->gmarks) + n, c->spec_nodes,
+    launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, c->sc,
+                     P<PhysNode>(c->pnodes), P<int32_t>(c->level_list), P<uint32_t>(c->end_list),
+                     P<int32_t>(c->gmarks), P<uint32_t>(c->gmarks) + n, c->spec_nodes,
                      c->spec_int, s);
     c->spec_emit = true;
   } else if (!c->physics && c->nodes.p && c->heads.p && ci > 0) {
     c->spec_nodes = std::min(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead)));
     c->spec_int = ci;
-    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, sy, c->sc,
-                P<RefNode>(c->nodes), P<WalkHead>(c->heads), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val),
-                P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), c->spec_nodes, c->spec_int, s);
+    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, sy, c->sc,
+                P<RefNode>(c->nodes), P<WalkHead>(c->heads), P<int32_t>(c->level_list), P<uint32_t>(c->end_list),
+                c->spec_nodes, c->spec_int, s);
     c->spec_emit = true;
   }
   *skey_out = skey;
@@ -362,7 +358,7 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
 // Needs the scalars of the last build on the host.
 nbx_status ref_layout(nbx_ctx* c, bool emitted) {
   const int64_t n = c->n, total = c->total_nodes;
-  const int64_t m = (uint32_t)c->hsc->internal_nodes;
+  const int64_t m = internal_count(*c->hsc);
   hipStream_t s = c->stream;
   // The walk reads node records at 32-bit byte offsets, and one record past
   // the last node (a successor it loads but never uses).
@@ -371,16 +367,10 @@ nbx_status ref_layout(nbx_ctx* c, bool emitted) {
   uint32_t* scr = P<uint32_t>(c->scratch);
   RefNode* nodes = P<RefNode>(c->nodes);
   if (!emitted)
-    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), sysa(c),
-                c->sc, nodes, P<WalkHead>(c->heads), P<uint32_t>(c->st_key), P<uint32_t>(c->st_val),
-                P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
+    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), view(c, c->cur), sysa(c),
+                c->sc, nodes, P<WalkHead>(c->heads), P<int32_t>(c->level_list), P<uint32_t>(c->end_list),
                 std::min(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead))), cap_int(c), s);
-  const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
-                                 P<uint32_t>(c->st_val2), m, 8, scr, s);
-  const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
-                                 P<uint32_t>(c->en_val2), m, 8, scr, s);
-  launch_pair(m, P<uint32_t>(sa ? c->st_val2 : c->st_val), P<uint32_t>(ea ? c->en_val2 : c->en_val), nodes,
-              P<int32_t>(c->level_list), s);
+  launch_pair(m, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes, s);
   return NBX_OK;
 }
 
@@ -394,7 +384,6 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
-  uint32_t* tot_int = reinterpret_cast<uint32_t*>(&c->sc->internal_nodes);
   HIPCHK(c, hipEventSynchronize(c->ev_scalars));
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
 
@@ -436,10 +425,9 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     skey = P<uint64_t>(c->key[0]);
     skey2 = P<uint64_t>(c->key2[0]);
     launch_fill_scalars(c->sc, 1, s);
-    launch_lcp_count(n, skey, skey2, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e),
-                     P<uint32_t>(c->lvl_part), c->sc, s);
+    launch_lcp_count(n, skey, skey2, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
+                     c->sc, s);
     scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
-    scan_u32(P<uint32_t>(c->cnt_e), P<uint32_t>(c->cnt_e), n, tot_int, scr, s);
     NBXCHK(sync_scalars(c));
     if (c->hsc->coincident > 0)
       return fail(c, NBX_ERR_DEPTH,
@@ -451,16 +439,14 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   launch_sys_roots(sy, P<uint32_t>(c->cnt_s), c->sc, s);
 
   const int64_t total = (uint32_t)c->hsc->total_nodes;
-  const int64_t m = (uint32_t)c->hsc->internal_nodes;
+  const int64_t m = internal_count(*c->hsc);
   c->total_nodes = total;
   c->max_depth = c->hsc->max_leaf_depth;
   // The reference walk reads 128-byte node records at 32-bit byte offsets.
   if (total >= (int64_t)1 << 25)
     return fail(c, NBX_ERR_ARG, "the tree has " + std::to_string(total) + " nodes; at most 33554431 are supported");
   const size_t mm = (size_t)std::max<int64_t>(m, 1) * 4;
-  for (DBuf* b : {&c->st_key, &c->st_val, &c->en_key, &c->en_val, &c->st_key2, &c->st_val2, &c->en_key2,
-                  &c->en_val2, &c->level_list})
-    NBXCHK(ensure(c, *b, mm));
+  for (DBuf* b : {&c->level_list, &c->end_list}) NBXCHK(ensure(c, *b, mm));
   NBXCHK(ensure(c, c->scratch,
                 std::max(sort_scratch_words(std::max(c->cap, m)), scan_scratch_words(std::max(c->cap, total))) * 4));
   int64_t off[kMaxDepthSlots + 1];
@@ -477,16 +463,11 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     const bool emitted = c->spec_emit && total <= (int64_t)c->spec_nodes && m <= (int64_t)c->spec_int;
     if (!emitted) {
       launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
-      launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), St, c->sc, pn,
-                       P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key),
-                       P<uint32_t>(c->en_val), gcnt, gmark, cap_words(c->pnodes, sizeof(PhysNode)), cap_int(c), s);
+      launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, c->sc, pn,
+                       P<int32_t>(c->level_list), P<uint32_t>(c->end_list), gcnt, gmark,
+                       cap_words(c->pnodes, sizeof(PhysNode)), cap_int(c), s);
     }
-    const bool sa = radix_sort_u32(P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->st_key2),
-                                   P<uint32_t>(c->st_val2), m, 8, scr, s);
-    const bool ea = radix_sort_u32(P<uint32_t>(c->en_key), P<uint32_t>(c->en_val), P<uint32_t>(c->en_key2),
-                                   P<uint32_t>(c->en_val2), m, 8, scr, s);
-    launch_phys_pair(m, P<uint32_t>(sa ? c->st_val2 : c->st_val), P<uint32_t>(ea ? c->en_val2 : c->en_val), pn,
-                     P<int32_t>(c->level_list), s);
+    launch_phys_pair(m, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), pn, s);
     record(c, EV_BUILD);
     for (int d = kMaxDepthSlots - 1; d >= 0; --d)
       if (c->hsc->level_count[d] > 0)
@@ -535,9 +516,11 @@ nbx_status refit_tree(nbx_ctx* c) {
   int32_t* gcnt = P<int32_t>(c->gmarks);
   uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
   PhysNode* pn = P<PhysNode>(c->pnodes);
-  launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_e), view(c, c->cur), c->sc, pn,
-                   P<uint32_t>(c->st_key), P<uint32_t>(c->st_val), P<uint32_t>(c->en_key), P<uint32_t>(c->en_val),
-                   gcnt, gmark, cap_words(c->pnodes, sizeof(PhysNode)), cap_int(c), s);
+  // leaf records from the current positions; the node lists of the last
+  // build stand (list capacity 0: nothing written to them)
+  launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), view(c, c->cur), c->sc, pn,
+                   P<int32_t>(c->level_list), P<uint32_t>(c->end_list), gcnt, gmark,
+                   cap_words(c->pnodes, sizeof(PhysNode)), 0u, s);
   record(c, EV_BUILD);
   int64_t off[kMaxDepthSlots + 1];
   off[0] = 0;

@@ -22,6 +22,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "layout.hpp"
 #include "nbx.h"
 #include "prims.hpp"
 #include "ref_kernels.hpp"
@@ -39,7 +40,7 @@ constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // Byte offsets of every buffer inside the array, for kNaBodies bodies and
 // kNaNodes nodes.
 struct Layout {
-  size_t sc, st[2][10], acc[3], key[2], idx[2], key2[2], lcp, cnt_s, cnt_e, part, sortscr, nodes, heads, ik[9];
+  size_t sc, st[2][10], acc[3], key[2], idx[2], key2[2], lcp, cnt_s, part, sortscr, nodes, heads, level_list, end_list;
   size_t root, dt, v2, soff, sroot, send, groups, boxes, box_off, box_part, total;
 };
 
@@ -58,7 +59,6 @@ constexpr Layout make_layout() {
   }
   L.lcp = take((size_t)kNaBodies * 4);
   L.cnt_s = take((size_t)kNaBodies * 4);
-  L.cnt_e = take((size_t)kNaBodies * 4);
   L.part = take((size_t)(((kNaBodies + kBlock - 1) / kBlock) * kPartW) * 4);
   const int64_t big = kNaBodies > kNaNodes ? kNaBodies : kNaNodes;
   const size_t sw = (size_t)(kSortBins * ((big + kSortTile - 1) / kSortTile) + kSortBins);
@@ -66,7 +66,8 @@ constexpr Layout make_layout() {
   L.sortscr = take((sw > cw ? sw : cw) * 4);
   L.nodes = take((size_t)(kNaNodes + 1) * sizeof(RefNode));   // one spare record for the walk
   L.heads = take((size_t)(kNaNodes + 1) * sizeof(WalkHead));
-  for (int k = 0; k < 9; ++k) L.ik[k] = take((size_t)kNaNodes * 4);
+  L.level_list = take((size_t)kNaNodes * 4);
+  L.end_list = take((size_t)kNaNodes * 4);
   L.root = take(4 * 8);
   L.dt = take(8);
   L.v2 = take(8);
@@ -199,13 +200,10 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   const uint64_t* skey = key[in1 ? 1 : 0];
   int32_t* lcp = A.at<int32_t>(kLayout.lcp);
   uint32_t* cnt_s = A.at<uint32_t>(kLayout.cnt_s);
-  uint32_t* cnt_e = A.at<uint32_t>(kLayout.cnt_e);
   uint32_t* part = A.at<uint32_t>(kLayout.part);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&sc->total_nodes);
-  uint32_t* tot_int = reinterpret_cast<uint32_t*>(&sc->internal_nodes);
-  launch_lcp_count(n, skey, nullptr, St.sys, lcp, cnt_s, cnt_e, part, sc, s);
+  launch_lcp_count(n, skey, nullptr, St.sys, lcp, cnt_s, part, sc, s);
   scan_u32(cnt_s, cnt_s, n, tot_nodes, scr, s);
-  scan_u32(cnt_e, cnt_e, n, tot_int, scr, s);
   DevScalars h;
   {
     const nbx_status r = read_scalars(sc, &h, s);
@@ -232,28 +230,23 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
     skey = key[0];
     skey2 = key2[0];
     launch_fill_scalars(sc, 1, s);
-    launch_lcp_count(n, skey, skey2, St.sys, lcp, cnt_s, cnt_e, part, sc, s);
+    launch_lcp_count(n, skey, skey2, St.sys, lcp, cnt_s, part, sc, s);
     scan_u32(cnt_s, cnt_s, n, tot_nodes, scr, s);
-    scan_u32(cnt_e, cnt_e, n, tot_int, scr, s);
     const nbx_status r = read_scalars(sc, &h, s);
     if (r != NBX_OK) return r;
     if (h.coincident > 0) return NBX_ERR_DEPTH;
   }
   const int64_t total = (uint32_t)h.total_nodes;
-  const int64_t mi = (uint32_t)h.internal_nodes;
+  const int64_t mi = internal_count(h);
   if (total > kNaNodes || mi > kNaNodes) return NBX_ERR_NOMEM;   // the tree does not fit the array
   launch_sys_roots(sy, cnt_s, sc, s);
   RefNode* nodes = A.at<RefNode>(kLayout.nodes);
-  uint32_t* ik[9];
-  for (int k = 0; k < 9; ++k) ik[k] = A.at<uint32_t>(kLayout.ik[k]);
-  // ik: st_key st_val en_key en_val st_key2 st_val2 en_key2 en_val2 level_list
   WalkHead* heads = A.at<WalkHead>(kLayout.heads);
-  launch_emit(n, lcp, cnt_s, cnt_e, St, sy, sc, nodes, heads, ik[0], ik[1], ik[2], ik[3], (uint32_t)(kNaNodes + 1),
+  int32_t* level_list = A.at<int32_t>(kLayout.level_list);
+  uint32_t* end_list = A.at<uint32_t>(kLayout.end_list);
+  launch_emit(n, lcp, cnt_s, part, St, sy, sc, nodes, heads, level_list, end_list, (uint32_t)(kNaNodes + 1),
               (uint32_t)kNaNodes, s);
-  const bool sa = radix_sort_u32(ik[0], ik[1], ik[4], ik[5], mi, 8, scr, s);
-  const bool ea = radix_sort_u32(ik[2], ik[3], ik[6], ik[7], mi, 8, scr, s);
-  int32_t* level_list = reinterpret_cast<int32_t*>(ik[8]);
-  launch_pair(mi, sa ? ik[5] : ik[1], ea ? ik[7] : ik[3], nodes, level_list, s);
+  launch_pair(mi, level_list, end_list, nodes, s);
   int64_t off = mi;
   for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
     const int64_t cnt = h.level_count[d];

@@ -32,6 +32,7 @@
 //   a = m d / r^3 - Q d / r^5 + (5/2) (d^T Q d) d / r^7,  r^2 = |d|^2 + eps^2
 // (the gradient of phi = -m/r - (1/2) x^T Q x / r^5 with x = -d).
 #include "common.hpp"
+#include "layout.hpp"
 #include "phys_kernels.hpp"
 #include "prims.hpp"
 
@@ -66,26 +67,30 @@ __device__ __forceinline__ void mark_group(int32_t f, int32_t c, int32_t* gcnt, 
 // pass.  A leaf that is a system's root is that system's walk group.
 __global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* __restrict__ lcp,
                                                       const uint32_t* __restrict__ pre_off,
-                                                      const uint32_t* __restrict__ end_off, StateArrays st,
+                                                      const uint32_t* __restrict__ part, StateArrays st,
                                                       const DevScalars* __restrict__ sc, PhysNode* __restrict__ pn,
-                                                      uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
-                                                      uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val,
+                                                      int32_t* __restrict__ level_list,
+                                                      uint32_t* __restrict__ end_list,
                                                       int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark,
                                                       uint32_t cap_nodes, uint32_t cap_int) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int li = lcp[i];
-  const int lp = i > 0 ? lcp[i - 1] : -1;
-  const uint32_t pre = pre_off[i];
+  const bool valid = i < n;
+  int li = -1, lp = -1;
+  uint32_t pre = 0, skipv = 0;
+  if (valid) {
+    li = lcp[i];
+    lp = i > 0 ? lcp[i - 1] : -1;
+    pre = pre_off[i];
+    skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
+  }
+  place_internal_nodes(valid, lp, li, pre, skipv, part + (int64_t)blockIdx.x * kPartW, sc, level_list, end_list,
+                       cap_int);
+  if (!valid) return;
   const int ns = li > lp ? li - lp : 0;
-  const uint32_t sbase = pre - (uint32_t)i;
   for (int t = 0; t < ns; ++t) {
     const uint32_t p = pre + t;
-    if (sbase + t < cap_int) {
-      st_key[sbase + t] = (uint32_t)(lp + 1 + t);
-      st_val[sbase + t] = p;
-    }
-    if (p < cap_nodes) pn[p].first = (int32_t)i;
+    if (p >= cap_nodes) break;
+    pn[p].first = (int32_t)i;
   }
   if (pre + ns < cap_nodes) {
     const uint32_t p = pre + ns;
@@ -102,27 +107,15 @@ __global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* 
     pn[p] = nd;
   }
   if (li < 0 && lp < 0) mark_group((int32_t)i, 1, gcnt, gmark);
-  const int ne = lp > li ? lp - li : 0;
-  if (ne) {
-    const uint32_t skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
-    const uint32_t eo = end_off[i];
-    for (int t = 0; t < ne && eo + t < cap_int; ++t) {
-      en_key[eo + t] = (uint32_t)(li + 1 + t);
-      en_val[eo + t] = skipv;
-    }
-  }
 }
 
-// The k-th internal node of depth d in start order is the k-th in end order:
-// its skip pointer, and the per-depth lists of internal nodes.
-__global__ __launch_bounds__(kBlock) void k_phys_pair(int64_t m, const uint32_t* __restrict__ st_val,
-                                                      const uint32_t* __restrict__ en_val,
-                                                      PhysNode* __restrict__ pn, int32_t* __restrict__ level_list) {
+// Skip pointer of the k-th internal node of the per-depth list.
+__global__ __launch_bounds__(kBlock) void k_phys_pair(int64_t m, const int32_t* __restrict__ level_list,
+                                                      const uint32_t* __restrict__ end_list,
+                                                      PhysNode* __restrict__ pn) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= m) return;
-  const uint32_t p = st_val[k];
-  pn[p].skip = (int32_t)en_val[k];
-  level_list[k] = (int32_t)p;
+  pn[level_list[k]].skip = (int32_t)end_list[k];
 }
 
 // Internal nodes of one depth from their children (FP64), and the walk
@@ -656,19 +649,18 @@ __global__ __launch_bounds__(kBlock) void k_group_compact(int64_t n, const int32
   if (c != 0) groups[gidx[i]] = Group{(int32_t)i, c, body_sys[i], 0};
 }
 
-void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
-                      const StateArrays& st, const DevScalars* sc, PhysNode* pn, uint32_t* st_key, uint32_t* st_val,
-                      uint32_t* en_key, uint32_t* en_val, int32_t* gcnt, uint32_t* gmark, uint32_t cap_nodes,
-                      uint32_t cap_int, hipStream_t s) {
+void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* part,
+                      const StateArrays& st, const DevScalars* sc, PhysNode* pn, int32_t* level_list,
+                      uint32_t* end_list, int32_t* gcnt, uint32_t* gmark, uint32_t cap_nodes, uint32_t cap_int,
+                      hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_phys_emit, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sc,
-                       pn, st_key, st_val, en_key, en_val, gcnt, gmark, cap_nodes, cap_int);
+    hipLaunchKernelGGL(k_phys_emit, dim3(grid_of(n, kBlock)), dim3(kBlock), 0, s, n, lcp, pre_off, part, st, sc, pn,
+                       level_list, end_list, gcnt, gmark, cap_nodes, cap_int);
 }
 
-void launch_phys_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, PhysNode* pn, int32_t* level_list,
-                      hipStream_t s) {
+void launch_phys_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn, hipStream_t s) {
   if (m > 0)
-    hipLaunchKernelGGL(k_phys_pair, dim3(grid_of(m, kBlock)), dim3(kBlock), 0, s, m, st_val, en_val, pn, level_list);
+    hipLaunchKernelGGL(k_phys_pair, dim3(grid_of(m, kBlock)), dim3(kBlock), 0, s, m, level_list, end_list, pn);
 }
 
 void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, int depth, PhysNode* pn, void* aux,

@@ -46,15 +46,15 @@ constexpr int kPhysAuxBytes = 128;  // FP64 moments per node (mass, bmax, Q, box
 // Walk groups are marked per first body in gcnt (body count) and gmark
 // (1), both zeroed beforehand, then compacted in body order.
 // Node layout from the sorted keys (the arguments of launch_emit, capacities
-// included): leaf records complete, internal nodes' first body, the
-// depth-sort keys; a leaf that is a system root is a one-body group.
-void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
-                      const StateArrays& st, const DevScalars* sc, PhysNode* pn, uint32_t* st_key, uint32_t* st_val,
-                      uint32_t* en_key, uint32_t* en_val, int32_t* gcnt, uint32_t* gmark, uint32_t cap_nodes,
-                      uint32_t cap_int, hipStream_t s);
-// Skip pointers and per-depth internal node lists from the two depth sorts.
-void launch_phys_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, PhysNode* pn, int32_t* level_list,
+// included): leaf records complete, internal nodes' first body, their places
+// in the per-depth node list and the list of skip targets; a leaf that is a
+// system root is a one-body group.
+void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* part,
+                      const StateArrays& st, const DevScalars* sc, PhysNode* pn, int32_t* level_list,
+                      uint32_t* end_list, int32_t* gcnt, uint32_t* gmark, uint32_t cap_nodes, uint32_t cap_int,
                       hipStream_t s);
+// Skip pointers of the m internal nodes from the two lists.
+void launch_phys_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn, hipStream_t s);
 // Internal nodes of one depth from their children (FP64 accumulation), and
 // the walk groups of at most group_size bodies each node packs (see
 // k_phys_upward).  `aux`: kPhysAuxBytes per node of FP64 moments (mass,

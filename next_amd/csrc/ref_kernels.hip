@@ -19,6 +19,7 @@
 //         acceptance test (octree.h:133-179), so every lane accumulates the
 //         same terms in the same order as the recursive reference.
 #include "common.hpp"
+#include "layout.hpp"
 #include "prims.hpp"
 #include "ref_kernels.hpp"
 
@@ -257,13 +258,13 @@ __device__ __forceinline__ int lcp_pair(const uint64_t* key, const uint64_t* key
 // lcp[i] = digits shared by sorted bodies i and i+1 (-1 past the end).
 // Body i starts the internal nodes at depths (lcp[i-1], lcp[i]] and its leaf;
 // it ends the internal nodes at depths (lcp[i], lcp[i-1]].  Each block also
-// writes one row of partial counts (internal nodes per depth, deepest leaf,
-// equal-key pairs) for k_lcp_reduce: plain stores, no global atomics.
+// writes one row of partial counts (internal nodes started and ended per
+// depth, deepest leaf, equal-key pairs) for k_lcp_scan: plain stores, no
+// global atomics.
 __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t* __restrict__ key,
                                                       const uint64_t* __restrict__ key2,
                                                       const int32_t* __restrict__ sys, int32_t* __restrict__ lcp,
-                                                      uint32_t* __restrict__ cnt_s, uint32_t* __restrict__ cnt_e,
-                                                      uint32_t* __restrict__ part) {
+                                                      uint32_t* __restrict__ cnt_s, uint32_t* __restrict__ part) {
   __shared__ uint32_t hist[kPartW];
   if (threadIdx.x < kPartW) hist[threadIdx.x] = 0;
   __syncthreads();
@@ -274,8 +275,8 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t*
     const int lp = (i > 0) ? lcp_pair(key, key2, sys, i - 1, eq_p) : -1;
     lcp[i] = li;
     cnt_s[i] = 1u + (uint32_t)(li > lp ? li - lp : 0);
-    cnt_e[i] = (uint32_t)(lp > li ? lp - li : 0);
     for (int d = lp + 1; d <= li; ++d) atomicAdd(&hist[d], 1u);
+    for (int d = li + 1; d <= lp; ++d) atomicAdd(&hist[kPartEnds + d], 1u);
     atomicMax(&hist[kPartMaxDepth], (uint32_t)(1 + (li > lp ? li : lp)));
     if (eq_i) atomicAdd(&hist[kPartEqual], 1u);
   }
@@ -283,11 +284,28 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t*
   if (threadIdx.x < kPartW) part[(int64_t)blockIdx.x * kPartW + threadIdx.x] = hist[threadIdx.x];
 }
 
-// Column b of the partial rows: one block per column.
-__global__ __launch_bounds__(kBlock) void k_lcp_reduce(const uint32_t* __restrict__ part, int64_t rows, int has_key2,
-                                                       DevScalars* sc) {
-  __shared__ uint32_t red[kBlock / kWave];
+// Column `col` of the partial rows, one block per column: a per-depth column
+// becomes its exclusive prefix over the blocks (in place) and its total is
+// the node count of that depth; the last two columns reduce to the deepest
+// leaf and the equal-key pairs.
+__global__ __launch_bounds__(kBlock) void k_lcp_scan(uint32_t* __restrict__ part, int64_t rows, int has_key2,
+                                                     DevScalars* sc) {
+  __shared__ uint32_t scr[kBlock / kWave + 1];
   const int col = blockIdx.x;
+  if (col < kPartMaxDepth) {
+    uint32_t carry = 0;
+    for (int64_t c = 0; c < rows; c += kBlock) {
+      const int64_t r = c + threadIdx.x;
+      const uint32_t v = r < rows ? part[r * kPartW + col] : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan_u32<kBlock>(v, scr, &tot);
+      if (r < rows) part[r * kPartW + col] = carry + ex;
+      carry += tot;
+    }
+    if (threadIdx.x == 0 && col < kPartEnds) sc->level_count[col] = (int32_t)carry;
+    return;
+  }
+  __shared__ uint32_t red[kBlock / kWave];
   const bool is_max = col == kPartMaxDepth;
   uint32_t v = 0;
   for (int64_t r = threadIdx.x; r < rows; r += kBlock) {
@@ -303,8 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_lcp_reduce(const uint32_t* __restric
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kBlock / kWave; ++w) v = is_max ? (red[w] > v ? red[w] : v) : v + red[w];
-    if (col < kMaxDepthSlots) sc->level_count[col] = (int32_t)v;
-    else if (is_max) sc->max_leaf_depth = (int32_t)v;
+    if (is_max) sc->max_leaf_depth = (int32_t)v;
     else if (has_key2) sc->coincident = (int32_t)v;
     else sc->collisions = (int32_t)v;
   }
@@ -322,28 +339,30 @@ __global__ __launch_bounds__(kBlock) void k_sys_roots(SysArrays sy, const uint32
 
 __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __restrict__ lcp,
                                                  const uint32_t* __restrict__ pre_off,
-                                                 const uint32_t* __restrict__ end_off, StateArrays st,
-                                                 SysArrays sy, const DevScalars* __restrict__ sc,
-                                                 RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
-                                                 uint32_t* __restrict__ st_key, uint32_t* __restrict__ st_val,
-                                                 uint32_t* __restrict__ en_key, uint32_t* __restrict__ en_val,
-                                                 uint32_t cap_nodes, uint32_t cap_int) {
+                                                 const uint32_t* __restrict__ part, StateArrays st, SysArrays sy,
+                                                 const DevScalars* __restrict__ sc, RefNode* __restrict__ nodes,
+                                                 WalkHead* __restrict__ heads, int32_t* __restrict__ level_list,
+                                                 uint32_t* __restrict__ end_list, uint32_t cap_nodes,
+                                                 uint32_t cap_int) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int li = lcp[i];
-  const int lp = i > 0 ? lcp[i - 1] : -1;
-  const uint32_t pre = pre_off[i];
+  const bool valid = i < n;
+  int li = -1, lp = -1;
+  uint32_t pre = 0, skipv = 0;
+  if (valid) {
+    li = lcp[i];
+    lp = i > 0 ? lcp[i - 1] : -1;
+    pre = pre_off[i];
+    skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
+  }
+  place_internal_nodes(valid, lp, li, pre, skipv, part + (int64_t)blockIdx.x * kPartW, sc, level_list, end_list,
+                       cap_int);
+  if (!valid) return;
   const double s0 = sy.root[4 * st.sys[i] + 3];
   const int ns = li > lp ? li - lp : 0;
-  const uint32_t sbase = pre - (uint32_t)i;
   for (int t = 0; t < ns; ++t) {
     const int d = lp + 1 + t;
     const uint32_t p = pre + t;
-    if (sbase + t < cap_int) {
-      st_key[sbase + t] = (uint32_t)d;
-      st_val[sbase + t] = p;
-    }
-    if (p >= cap_nodes) continue;
+    if (p >= cap_nodes) break;
     // mass, centre of mass, softening numerator and quadrupole come from
     // k_upward, the skip pointer from k_pair
     RefNode& nd = nodes[p];
@@ -371,26 +390,14 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
     // a leaf is accepted whatever its distance: no threshold
     heads[p] = WalkHead{nd.cx, nd.cy, nd.cz, 0.0f, m == 0.0 ? (int32_t)(p + 1) | kLinkZeroMass : ~(int32_t)i};
   }
-  const int ne = lp > li ? lp - li : 0;
-  if (ne) {
-    const uint32_t skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
-    const uint32_t eo = end_off[i];
-    for (int t = 0; t < ne && eo + t < cap_int; ++t) {
-      en_key[eo + t] = (uint32_t)(li + 1 + t);
-      en_val[eo + t] = skipv;
-    }
-  }
 }
 
-// The k-th internal node of depth d in start order is the k-th in end order.
-__global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const uint32_t* __restrict__ st_val,
-                                                 const uint32_t* __restrict__ en_val, RefNode* __restrict__ nodes,
-                                                 int32_t* __restrict__ level_list) {
+// Skip pointer of the k-th internal node of the per-depth list.
+__global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const int32_t* __restrict__ level_list,
+                                                 const uint32_t* __restrict__ end_list, RefNode* __restrict__ nodes) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= m) return;
-  const uint32_t p = st_val[k];
-  nodes[p].skip = (int32_t)en_val[k];
-  level_list[k] = (int32_t)p;
+  nodes[level_list[k]].skip = (int32_t)end_list[k];
 }
 
 // Mass, centre of mass and one-level softened quadrupole of the internal
@@ -900,12 +907,11 @@ void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src
   if (n > 0) hipLaunchKernelGGL(k_gather_state, dim3(grid_of(n)), dim3(kBlock), 0, s, n, perm, src, dst);
 }
 void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, const int32_t* sys, int32_t* lcp,
-                      uint32_t* cnt_s, uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s) {
+                      uint32_t* cnt_s, uint32_t* part, DevScalars* sc, hipStream_t s) {
   if (n <= 0) return;
   const int64_t rows = grid_of(n);
-  hipLaunchKernelGGL(k_lcp_count, dim3((unsigned)rows), dim3(kBlock), 0, s, n, key, key2, sys, lcp, cnt_s, cnt_e,
-                     part);
-  hipLaunchKernelGGL(k_lcp_reduce, dim3(kPartW), dim3(kBlock), 0, s, part, rows, key2 ? 1 : 0, sc);
+  hipLaunchKernelGGL(k_lcp_count, dim3((unsigned)rows), dim3(kBlock), 0, s, n, key, key2, sys, lcp, cnt_s, part);
+  hipLaunchKernelGGL(k_lcp_scan, dim3(kPartW), dim3(kBlock), 0, s, part, rows, key2 ? 1 : 0, sc);
 }
 void launch_remove_map(int64_t n_new, int64_t cut, int64_t len, uint32_t* perm, hipStream_t s) {
   if (n_new > 0) hipLaunchKernelGGL(k_remove_map, dim3(grid_of(n_new)), dim3(kBlock), 0, s, n_new, cut, len, perm);
@@ -919,17 +925,15 @@ void launch_gather_f64(int64_t n, const uint32_t* perm, const double* src, doubl
 void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_sys_roots, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, pre_off, sc);
 }
-void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
+void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* part,
                  const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, WalkHead* heads,
-                 uint32_t* st_key, uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes,
-                 uint32_t cap_int, hipStream_t s) {
+                 int32_t* level_list, uint32_t* end_list, uint32_t cap_nodes, uint32_t cap_int, hipStream_t s) {
   if (n > 0)
-    hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, end_off, st, sy, sc, nodes,
-                       heads, st_key, st_val, en_key, en_val, cap_nodes, cap_int);
+    hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, part, st, sy, sc, nodes, heads,
+                       level_list, end_list, cap_nodes, cap_int);
 }
-void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
-                 hipStream_t s) {
-  if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, st_val, en_val, nodes, level_list);
+void launch_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes, hipStream_t s) {
+  if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, level_list, end_list, nodes);
 }
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
                    double theta, hipStream_t s) {

@@ -18,10 +18,14 @@ struct StateArrays {
 
 constexpr int kBlock = 256;
 constexpr int kBoxSpan = 2048;      // bodies per block of the bounding-box reduction
-// Row of per-block partial counts written by the lcp kernel.
-constexpr int kPartMaxDepth = kMaxDepthSlots;      // column: deepest leaf
-constexpr int kPartEqual = kMaxDepthSlots + 1;     // column: equal-key pairs
-constexpr int kPartW = kMaxDepthSlots + 2;
+// Row of per-block partial counts written by the lcp kernel: internal nodes
+// started per depth (columns 0..63), ended per depth (64..127), deepest leaf,
+// equal-key pairs.  k_lcp_scan turns the per-depth columns into exclusive
+// prefixes over the blocks.
+constexpr int kPartEnds = kMaxDepthSlots;
+constexpr int kPartMaxDepth = 2 * kMaxDepthSlots;  // column: deepest leaf
+constexpr int kPartEqual = 2 * kMaxDepthSlots + 1; // column: equal-key pairs
+constexpr int kPartW = 2 * kMaxDepthSlots + 2;
 inline int64_t lcp_part_words(int64_t n) { return ((n + kBlock - 1) / kBlock) * kPartW; }
 
 void launch_fill_scalars(DevScalars* sc, int what, hipStream_t s);  // what: 1 build counters, 4 err, 8 walk counters
@@ -39,22 +43,26 @@ void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint
 void launch_gather_u32(int64_t n, const uint32_t* perm, const uint32_t* src, uint32_t* dst, hipStream_t s);
 void launch_gather_state(int64_t n, const uint32_t* perm, const StateArrays& src, const StateArrays& dst,
                          hipStream_t s);
+// lcp of the sorted keys, each body's node count (its started internal
+// nodes and its leaf) in cnt_s, the per-block rows of per-depth counts
+// (scanned over the blocks) in part, and in sc the internal nodes per depth,
+// the deepest leaf and the equal-key pairs.
 void launch_lcp_count(int64_t n, const uint64_t* key, const uint64_t* key2, const int32_t* sys, int32_t* lcp,
-                      uint32_t* cnt_s, uint32_t* cnt_e, uint32_t* part, DevScalars* sc, hipStream_t s);
+                      uint32_t* cnt_s, uint32_t* part, DevScalars* sc, hipStream_t s);
 void launch_sys_roots(const SysArrays& sy, const uint32_t* pre_off, const DevScalars* sc, hipStream_t s);
 void launch_remove_map(int64_t n_new, int64_t cut, int64_t len, uint32_t* perm, hipStream_t s);
 void launch_relabel(int64_t n, int32_t* sys, int32_t removed, hipStream_t s);
 void launch_gather_f64(int64_t n, const uint32_t* perm, const double* src, double* dst, hipStream_t s);
-// Node records (and the walk heads of the leaves) and depth-sort keys from
-// the sorted keys' lcp and the two scans; writes only records below cap_nodes and key entries below
-// cap_int (the buffers' capacities), so it can run before the host knows the
-// node counts.
-void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* end_off,
+// Node records (and the walk heads of the leaves) from the sorted keys' lcp
+// and the scan of cnt_s (pre_off), and every internal node's place in the
+// per-depth node list and the list of skip targets (layout.hpp); writes
+// records below cap_nodes and list entries below cap_int only (the buffers'
+// capacities), so it can run before the host knows the node counts.
+void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const uint32_t* part,
                  const StateArrays& st, const SysArrays& sy, const DevScalars* sc, RefNode* nodes, WalkHead* heads,
-                 uint32_t* st_key, uint32_t* st_val, uint32_t* en_key, uint32_t* en_val, uint32_t cap_nodes,
-                 uint32_t cap_int, hipStream_t s);
-void launch_pair(int64_t m, const uint32_t* st_val, const uint32_t* en_val, RefNode* nodes, int32_t* level_list,
-                 hipStream_t s);
+                 int32_t* level_list, uint32_t* end_list, uint32_t cap_nodes, uint32_t cap_int, hipStream_t s);
+// Skip pointers of the m internal nodes from the two lists.
+void launch_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes, hipStream_t s);
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
                    double theta, hipStream_t s);
 void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s);
