@@ -99,6 +99,12 @@ typedef struct nbx_timing {
   int64_t n_open;      /* opened cells, per target (reference) or per group (physics) (if counting) */
   int64_t n_iter;      /* walk traversal steps summed over waves (if counting) */
   int64_t n_eval;      /* interaction-evaluation steps summed over waves (if counting) */
+  int64_t graph_steps;    /* steps replayed from a captured graph (nbx_set_graph) */
+  int64_t graph_launches; /* graph launches */
+  int64_t graph_nodes;    /* kernels in the captured period of steps */
+  int64_t graph_period;   /* steps in the captured period */
+  int64_t graph_aborts;   /* captured runs redone step by step (a build did not fit) */
+  int64_t host_waits;     /* host waits on the device inside steps and runs */
 } nbx_timing;
 
 /* Fill *cfg with NEXT's settings: device 0, reference mode, FP64, theta 0.5,
@@ -234,6 +240,20 @@ nbx_status nbx_energy_systems(nbx_ctx* ctx, int source, double* out);
  * the call leaves behind is the one it started from. */
 nbx_status nbx_autotune(nbx_ctx* ctx, const int32_t* leaf, const int32_t* group, int32_t ncand, double dt,
                         int32_t steps, double* ms_out);
+
+/* Captured runs (no counterpart in NEXT).  With on = 1, nbx_run replays its
+ * steps from a HIP graph of a period of steps (2, or 2k in physics mode with
+ * the tree rebuilt every k-th step) whose tree builds never read counts back
+ * to the host: buffers and launches are sized from the last build with
+ * headroom, and a build that does not fit them (or needs the two-word keys,
+ * or meets a non-finite position) raises a device flag after which the
+ * run's tree kernels do nothing.  One host wait per run checks the flag; if
+ * it is set, the state the run started from is restored and the run is made
+ * step by step instead, so results are those of nbx_run without capture,
+ * bit for bit, either way.  Timings per phase are not recorded inside a
+ * captured run.  on = 2 sizes the bounded builds without headroom (a test of
+ * the fallback); 0 turns capture off.  NBX_ERR_ARG for other values. */
+nbx_status nbx_set_graph(nbx_ctx* ctx, int on);
 
 /* Accumulated timings and counters. */
 nbx_status nbx_get_timing(nbx_ctx* ctx, nbx_timing* out);

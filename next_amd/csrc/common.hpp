@@ -56,8 +56,20 @@ struct DevScalars {
   unsigned long long n_open, n_iter;    // opened nodes, wave traversal iterations
   unsigned long long n_eval;            // wave evaluation steps (deferred interactions)
   uint32_t tree_groups;                 // walk groups built from the tree (physics mode)
-  uint32_t pad_;
+  // Builds that never read counts back to the host (captured steps): each
+  // depth's first entry in the node list, and a sticky flag set when a
+  // build does not fit what its launches were sized for (kAbort*), after
+  // which the tree kernels of the run do nothing.
+  int32_t abort_flags;
+  int32_t level_off[kMaxDepthSlots];
+  int32_t step_count;                   // next entry of the dt log of a captured run
+  int32_t pad_;
 };
+
+constexpr int32_t kAbortCapacity = 1;   // nodes or node-list entries beyond the buffers
+constexpr int32_t kAbortDepth = 2;      // deeper than the levels launched
+constexpr int32_t kAbortCollision = 4;  // bodies sharing 21 digits: the two-word key path is needed
+constexpr int32_t kAbortNonFinite = 8;  // a position is NaN or infinite
 
 // Per-system arrays: a context holds one or more independent particle
 // systems, stored one after another in key order.

@@ -105,6 +105,31 @@ struct nbx_ctx {
   bool spec_emit = false;
   uint32_t spec_nodes = 0, spec_int = 0;
   nbx_timing tim{};
+  // Captured runs (nbx_set_graph): a period of steps whose tree builds never
+  // read counts back, replayed from a HIP graph.
+  int graph = 0;              // 0 off, 1 on, 2 on without headroom in the bounded sizes
+  uint64_t gen = 0;           // bumped when a buffer moves or the systems or settings change
+  hipGraphExec_t gexec = nullptr;
+  struct GraphKey {
+    uint64_t gen;
+    double base_dt;
+    int adaptive, period;
+    int64_t cap_records, cap_list, groups_bound;
+    int dmax;
+    unsigned grid_up, grid_pair;
+    bool operator==(const GraphKey& o) const {
+      return gen == o.gen && base_dt == o.base_dt && adaptive == o.adaptive && period == o.period &&
+             cap_records == o.cap_records && cap_list == o.cap_list && groups_bound == o.groups_bound &&
+             dmax == o.dmax && grid_up == o.grid_up && grid_pair == o.grid_pair;
+    }
+  } gkey{};
+  int64_t gnodes = 0;         // kernels in the captured period
+  // sizes of the bounded build (launch_check's limits and the fixed grids)
+  int64_t b_cap_records = 0, b_cap_list = 0;
+  int b_dmax = -1;
+  unsigned b_grid_up = 1, b_grid_pair = 1;
+  // the state a captured run starts from, restored if one of its builds aborts
+  DBuf snap[kStateArrays], snap_acc[3], snap_amag;
 };
 
 namespace {
@@ -151,11 +176,12 @@ nbx_status ensure(nbx_ctx* c, DBuf& b, size_t bytes, size_t keep = 0) {
   if (b.p) (void)hipFree(b.p);
   b.p = p;
   b.bytes = want;
+  ++c->gen;
   return NBX_OK;
 }
 
 void release(DBuf& b) {
-  if (b.p) (void)hipFree(b.p);
+  if (b.p) (void)hipFree(b.p);   // (callers that keep using the context bump its generation)
   b.p = nullptr;
   b.bytes = 0;
 }
@@ -169,8 +195,10 @@ void each_buf(nbx_ctx* c, F f) {
   for (DBuf* b : {&c->lcp, &c->cnt_s, &c->nodes, &c->heads, &c->level_list, &c->end_list, &c->scratch, &c->dts,
                   &c->lvl_part, &c->pnodes, &c->paux, &c->pot, &c->dacc, &c->epart, &c->eout, &c->s_root, &c->s_dt,
                   &c->s_v2, &c->s_soff, &c->s_sroot, &c->s_send, &c->groups, &c->boxes, &c->box_off, &c->box_part,
-                  &c->tgroups, &c->gmarks})
+                  &c->tgroups, &c->gmarks, &c->snap_amag})
     f(*b);
+  for (DBuf& b : c->snap) f(b);
+  for (DBuf& b : c->snap_acc) f(b);
 }
 
 size_t elem_bytes(int k) { return k < 7 ? 8 : 4; }
@@ -200,6 +228,7 @@ void record(nbx_ctx* c, int e) {
 nbx_status sync_scalars(nbx_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  ++c->tim.host_waits;
   return NBX_OK;
 }
 
@@ -266,6 +295,7 @@ nbx_status set_systems(nbx_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));   // the host vectors go out of scope
   c->ngroups = (int64_t)groups.size();
   c->nboxes = (int)boxes.size();
+  ++c->gen;
   c->acc_valid = false;
   c->tree_valid = false;
   c->layout_valid = false;
@@ -281,7 +311,9 @@ int sys_bits(int nsys) {
 // First half of a tree build (launches only): bounding boxes, keys, sort,
 // gather of the state into key order (flips c->cur), lcp, node counts and
 // their scan, and the copy of the device scalars to the host mirror.
-nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
+// Bounding boxes, keys, sort by key and then system, gather of the state
+// into key order (flips c->cur): the keys of the new order.
+const uint64_t* sort_bodies(nbx_ctx* c) {
   const int64_t n = c->n;
   const int S = (int)c->sys_n.size();
   hipStream_t s = c->stream;
@@ -318,7 +350,16 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
     skey = P<uint64_t>(c->key[0]);
   }
   record(c, EV_SORT);
+  return skey;
+}
 
+nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
+  const int64_t n = c->n;
+  hipStream_t s = c->stream;
+  const uint64_t* skey = sort_bodies(c);
+  SysArrays sy = sysa(c);
+  StateArrays St = view(c, c->cur);
+  uint32_t* scr = P<uint32_t>(c->scratch);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
   launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
                    c->sc, s);
@@ -385,6 +426,7 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   uint32_t* scr = P<uint32_t>(c->scratch);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
   HIPCHK(c, hipEventSynchronize(c->ev_scalars));
+  ++c->tim.host_waits;
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
 
   const uint64_t* skey2 = nullptr;
@@ -631,6 +673,225 @@ nbx_status step_device(nbx_ctx* c) {
   return finish_timing(c, true);
 }
 
+// ---- captured runs: steps whose tree builds keep their counts on the device
+
+// Steps in a captured period: the state and the |a| arrays are double
+// buffered and flip at every build, so the period holds an even number of
+// builds and starts with one.
+int graph_period(const nbx_ctx* c) { return c->physics ? 2 * c->rebuild_every : 2; }
+
+// Limits and grids of the bounded builds, from the last build: kept while
+// the last build fits them with a margin, else sized with a quarter of
+// headroom (no headroom with graph == 2).  The buffers are grown to match.
+nbx_status prepare_bounded(nbx_ctx* c, int32_t nsteps) {
+  const bool tight = c->graph == 2;
+  const int64_t n = c->n, total = c->total_nodes, m = internal_count(*c->hsc);
+  int64_t maxlvl = 1;
+  for (int d = 0; d < kMaxDepthSlots; ++d) maxlvl = std::max<int64_t>(maxlvl, (uint32_t)c->hsc->level_count[d]);
+  const int dneed = c->max_depth - 1;                      // deepest internal node of the last build
+  const bool keep = !tight && c->b_cap_records >= total + total / 16 + 64 && c->b_cap_list >= m + m / 16 + 64 &&
+                    c->b_dmax >= dneed + 1 && (int64_t)c->b_grid_up * kBlock >= maxlvl;
+  if (!keep) {
+    const int64_t rec = tight ? total + 1 : total + total / 4 + 1024;
+    const int64_t lst = tight ? std::max<int64_t>(m, 1) : m + m / 4 + 1024;
+    if (c->physics) {
+      NBXCHK(ensure(c, c->pnodes, (size_t)rec * sizeof(PhysNode)));
+      NBXCHK(ensure(c, c->paux, (size_t)rec * kPhysAuxBytes));
+    } else {
+      NBXCHK(ensure(c, c->nodes, (size_t)rec * sizeof(RefNode)));
+      NBXCHK(ensure(c, c->heads, (size_t)rec * sizeof(WalkHead)));
+    }
+    NBXCHK(ensure(c, c->level_list, (size_t)lst * 4));
+    NBXCHK(ensure(c, c->end_list, (size_t)lst * 4));
+    c->b_dmax = tight ? dneed : std::min(dneed + 4, kMaxLevels);
+    const int64_t up = tight ? maxlvl : maxlvl + maxlvl / 4;
+    c->b_grid_up = (unsigned)std::min<int64_t>(std::max<int64_t>(div_up(up, kBlock), 1), 4096);
+  }
+  c->b_cap_records = c->physics ? std::min<int64_t>(cap_words(c->pnodes, sizeof(PhysNode)), cap_words(c->paux, kPhysAuxBytes))
+                                : std::min<int64_t>(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead)));
+  c->b_cap_list = cap_int(c);
+  c->b_grid_pair = (unsigned)std::min<int64_t>(std::max<int64_t>(div_up(c->b_cap_list, kBlock), 1), 8192);
+  NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(std::max(c->cap, c->b_cap_list)),
+                                        scan_scratch_words(std::max(c->cap, c->b_cap_records))) * 4));
+  NBXCHK(ensure(c, c->dts, (size_t)std::max(nsteps, 1) * 8));
+  if (c->physics) {
+    NBXCHK(ensure(c, c->tgroups, (size_t)n * sizeof(Group)));
+    NBXCHK(ensure(c, c->gmarks, (size_t)n * 8));
+    const int64_t last = c->hsc->tree_groups;
+    if (tight || c->groups_bound < last + last / 16 + 64)
+      c->groups_bound = std::min<int64_t>(tight ? std::max<int64_t>(last, 1) : last + last / 4 + 64, n);
+    if (c->alpha > 0.0)
+      for (DBuf& b : c->amag) NBXCHK(ensure(c, b, (size_t)n * sizeof(float)));
+  }
+  return NBX_OK;
+}
+
+// Tree of the current positions without a host read-back: launch_check
+// compares the counts with the bounded sizes on the device and the tree
+// kernels after it do nothing if they do not fit.
+void build_bounded(nbx_ctx* c) {
+  const int64_t n = c->n;
+  hipStream_t s = c->stream;
+  const uint64_t* skey = sort_bodies(c);
+  SysArrays sy = sysa(c);
+  StateArrays St = view(c, c->cur);
+  uint32_t* scr = P<uint32_t>(c->scratch);
+  launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
+                   c->sc, s);
+  scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, reinterpret_cast<uint32_t*>(&c->sc->total_nodes), scr,
+           s);
+  launch_check(c->sc, c->b_cap_records, c->b_cap_list, c->b_dmax, s);
+  launch_sys_roots(sy, P<uint32_t>(c->cnt_s), c->sc, s);
+  const uint32_t cap_rec = (uint32_t)c->b_cap_records, cap_lst = (uint32_t)c->b_cap_list;
+  if (c->physics) {
+    int32_t* gcnt = P<int32_t>(c->gmarks);
+    uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
+    PhysNode* pn = P<PhysNode>(c->pnodes);
+    launch_fill_u32(2 * n, P<uint32_t>(c->gmarks), 0u, s);
+    launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, c->sc, pn,
+                     P<int32_t>(c->level_list), P<uint32_t>(c->end_list), gcnt, gmark, cap_rec, cap_lst, s);
+    launch_phys_pair_dev(c->b_grid_pair, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), pn, c->sc, s);
+    for (int d = c->b_dmax; d >= 0; --d)
+      launch_phys_upward_level(c->b_grid_up, d, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta,
+                               c->cfg.group_size, gcnt, gmark, c->sc, s);
+    launch_group_compact(n, gcnt, gmark, St.sys, P<Group>(c->tgroups), &c->sc->tree_groups, scr, s);
+    c->ntgroups = -1;                                     // the walk's grid is groups_bound
+    c->layout_valid = true;
+  } else {
+    RefNode* nodes = P<RefNode>(c->nodes);
+    launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, sy, c->sc, nodes,
+                P<WalkHead>(c->heads), P<int32_t>(c->level_list), P<uint32_t>(c->end_list), cap_rec, cap_lst, s);
+    launch_pair_dev(c->b_grid_pair, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes, c->sc, s);
+    for (int d = c->b_dmax; d >= 0; --d)
+      launch_upward_level(c->b_grid_up, d, P<int32_t>(c->level_list), nodes, P<WalkHead>(c->heads), c->cfg.theta,
+                          c->sc, s);
+  }
+  c->skey = skey;
+  c->skey2 = nullptr;
+  c->spec_emit = false;
+  c->tree_valid = true;
+}
+
+// Physics refit of the last bounded build's tree (its per-depth offsets are
+// on the device).
+void refit_bounded(nbx_ctx* c) {
+  const int64_t n = c->n;
+  hipStream_t s = c->stream;
+  int32_t* gcnt = P<int32_t>(c->gmarks);
+  uint32_t* gmark = P<uint32_t>(c->gmarks) + n;
+  PhysNode* pn = P<PhysNode>(c->pnodes);
+  launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), view(c, c->cur), c->sc, pn,
+                   P<int32_t>(c->level_list), P<uint32_t>(c->end_list), gcnt, gmark, (uint32_t)c->b_cap_records, 0u,
+                   s);
+  for (int d = c->b_dmax; d >= 0; --d)
+    launch_phys_upward_level(c->b_grid_up, d, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta,
+                             c->cfg.group_size, gcnt, gmark, c->sc, s);
+  c->tree_valid = true;
+}
+
+// One KDK step with no host wait: step_device's sequence with the bounded
+// build.  Needs the forces of the current positions and prepare_bounded.
+nbx_status step_bounded(nbx_ctx* c) {
+  const bool rel = c->physics && c->alpha > 0.0;
+  launch_kick_drift(c->n, view(c, c->cur), P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), sysa(c),
+                    rel ? P<float>(c->amag[c->acur]) : nullptr, c->stream);
+  c->acc_valid = false;
+  c->tree_valid = false;
+  c->amag_live = rel;
+  const bool refit = c->physics && c->since_build + 1 < c->rebuild_every;
+  if (refit) refit_bounded(c);
+  else build_bounded(c);
+  c->since_build = refit ? c->since_build + 1 : 0;
+  const nbx_status sw = walk(c, true);
+  c->amag_live = false;
+  return sw;
+}
+
+// The state a captured run starts from (positions, velocities and the rest
+// of the body arrays in their current slots, the cached forces and |a|).
+struct Snapshot {
+  int cur, acur, since_build;
+};
+
+nbx_status snapshot(nbx_ctx* c, Snapshot* sn) {
+  const size_t n = (size_t)c->n;
+  for (int k = 0; k < kStateArrays; ++k) {
+    NBXCHK(ensure(c, c->snap[k], n * elem_bytes(k)));
+    HIPCHK(c, hipMemcpyAsync(c->snap[k].p, c->st[c->cur][k].p, n * elem_bytes(k), hipMemcpyDeviceToDevice, c->stream));
+  }
+  for (int k = 0; k < 3; ++k) {
+    NBXCHK(ensure(c, c->snap_acc[k], n * 8));
+    HIPCHK(c, hipMemcpyAsync(c->snap_acc[k].p, c->acc[k].p, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (c->physics && c->alpha > 0.0) {
+    NBXCHK(ensure(c, c->snap_amag, n * sizeof(float)));
+    HIPCHK(c, hipMemcpyAsync(c->snap_amag.p, c->amag[c->acur].p, n * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->stream));
+  }
+  *sn = Snapshot{c->cur, c->acur, c->since_build};
+  return NBX_OK;
+}
+
+nbx_status restore(nbx_ctx* c, const Snapshot& sn) {
+  const size_t n = (size_t)c->n;
+  c->cur = sn.cur;
+  c->acur = sn.acur;
+  for (int k = 0; k < kStateArrays; ++k)
+    HIPCHK(c, hipMemcpyAsync(c->st[c->cur][k].p, c->snap[k].p, n * elem_bytes(k), hipMemcpyDeviceToDevice, c->stream));
+  for (int k = 0; k < 3; ++k)
+    HIPCHK(c, hipMemcpyAsync(c->acc[k].p, c->snap_acc[k].p, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (c->physics && c->alpha > 0.0)
+    HIPCHK(c, hipMemcpyAsync(c->amag[c->acur].p, c->snap_amag.p, n * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->stream));
+  c->since_build = sn.since_build;   // the next step is a build, as it was
+  c->acc_valid = true;
+  c->tree_valid = false;
+  c->layout_valid = false;
+  c->spec_emit = false;
+  return NBX_OK;
+}
+
+void drop_graph(nbx_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+}
+
+// A graph of one period of steps (each: dt from the speeds if adaptive, then
+// the step) captured from the context's stream; the host-side state (buffer
+// parity, refit phase) ends where it began.
+nbx_status capture(nbx_ctx* c, double base_dt, int period) {
+  drop_graph(c);
+  const int prof = c->prof;
+  c->prof = 0;
+  hipStreamCaptureMode mode = hipStreamCaptureModeThreadLocal;
+  HIPCHK(c, hipThreadExchangeStreamCaptureMode(&mode));
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  nbx_status st = e == hipSuccess ? NBX_OK : fail(c, NBX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+  SysArrays sy = sysa(c);
+  for (int p = 0; p < period && st == NBX_OK && e == hipSuccess; ++p) {
+    if (c->cfg.adaptive_dt) launch_dt(sy, base_dt, P<double>(c->dts), -1, c->sc, c->stream);
+    st = step_bounded(c);
+  }
+  hipGraph_t g = nullptr;
+  if (e == hipSuccess) e = hipStreamEndCapture(c->stream, &g);
+  (void)hipThreadExchangeStreamCaptureMode(&mode);
+  c->prof = prof;
+  if (st == NBX_OK && e != hipSuccess) st = fail(c, NBX_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+  if (st == NBX_OK) {
+    size_t nodes = 0;
+    e = hipGraphGetNodes(g, nullptr, &nodes);
+    if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) st = fail(c, NBX_ERR_HIP, std::string("graph: ") + hipGetErrorString(e));
+    c->gnodes = (int64_t)nodes;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  if (st != NBX_OK) {
+    (void)hipGetLastError();
+    drop_graph(c);
+  }
+  return st;
+}
+
 // A tree built from the current positions, for the inspection hooks.  A
 // refitted physics tree still has the cells of its last build, so it is
 // rebuilt here.
@@ -794,6 +1055,7 @@ void nbx_destroy(nbx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   each_buf(c, release);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->ev_scalars) (void)hipEventDestroy(c->ev_scalars);
@@ -864,6 +1126,7 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
   for (int32_t k = 0; k < ncand; ++k) {
     c->cfg.leaf_size = leaf[k];
     c->cfg.group_size = group[k];
+    ++c->gen;
     c->layout_valid = false;                               // groups of these sizes need a full build
     nbx_status st = nbx_run(c, dt, 1, nullptr);            // warm-up: buffers, first tree
     const auto t0 = std::chrono::steady_clock::now();
@@ -884,6 +1147,7 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
   c->cfg = keep;
   c->cfg.leaf_size = leaf[best];
   c->cfg.group_size = group[best];
+  ++c->gen;
   return NBX_OK;
 }
 
@@ -892,6 +1156,7 @@ nbx_status nbx_set_alpha(nbx_ctx* c, double alpha) {
   if (!(alpha >= 0.0 && alpha < INFINITY)) return fail(c, NBX_ERR_ARG, "alpha must be finite and >= 0");
   if (alpha > 0.0 && !c->physics) return fail(c, NBX_ERR_ARG, "the relative opening criterion is a physics-mode option");
   c->alpha = alpha;
+  ++c->gen;
   return NBX_OK;
 }
 
@@ -901,6 +1166,7 @@ nbx_status nbx_set_rebuild_interval(nbx_ctx* c, int32_t every) {
   if (every > 1 && !c->physics) return fail(c, NBX_ERR_ARG, "tree refits are a physics-mode option");
   c->rebuild_every = every;
   c->since_build = 0;
+  ++c->gen;
   return NBX_OK;
 }
 
@@ -911,6 +1177,7 @@ nbx_status nbx_set_theta(nbx_ctx* c, double theta) {
   c->layout_valid = false;
   c->cfg.theta = theta;
   c->acc_valid = false;
+  ++c->gen;
   return NBX_OK;
 }
 
@@ -1033,7 +1300,7 @@ nbx_status nbx_adaptive_dt(nbx_ctx* c, double base_dt, double* dt_out) {
   SysArrays sy = sysa(c);
   launch_fill_v2max(sy, c->stream);
   launch_maxv2(c->n, view(c, c->cur), sy, c->stream);
-  launch_dt(sy, base_dt, nullptr, 0, c->stream);
+  launch_dt(sy, base_dt, nullptr, 0, c->sc, c->stream);
   HIPCHK(c, hipMemcpyAsync(dt_out, c->s_dt.p, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return NBX_OK;
@@ -1058,19 +1325,81 @@ nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) 
   }
   NBXCHK(ensure(c, c->dts, (size_t)nsteps * 8));
   SysArrays sy = sysa(c);
-  if (c->cfg.adaptive_dt) {
-    launch_fill_v2max(sy, c->stream);
-    launch_maxv2(c->n, view(c, c->cur), sy, c->stream);
-  } else {
-    launch_set_dt(sy, base_dt, c->stream);
+  // dt of the first step from the current speeds (adaptive), or the fixed dt
+  auto first_dt = [&]() {
+    if (c->cfg.adaptive_dt) {
+      launch_fill_v2max(sy, c->stream);
+      launch_maxv2(c->n, view(c, c->cur), sy, c->stream);
+    } else {
+      launch_set_dt(sy, base_dt, c->stream);
+    }
+  };
+  first_dt();
+  int32_t s = 0;
+  const int period = graph_period(c);
+  if (c->graph && nsteps >= period) {
+    NBXCHK(ensure_accel(c));
+    // step by step up to a step that rebuilds the tree (physics refits)
+    for (; s < nsteps && c->physics && c->layout_valid && c->since_build + 1 < c->rebuild_every; ++s) {
+      if (c->cfg.adaptive_dt) launch_dt(sy, base_dt, P<double>(c->dts), s, c->sc, c->stream);
+      NBXCHK(step_device(c));
+    }
+    const int32_t reps = (nsteps - s) / period;
+    if (reps > 0) {
+      NBXCHK(prepare_bounded(c, nsteps));
+      Snapshot sn{};
+      NBXCHK(snapshot(c, &sn));
+      const nbx_ctx::GraphKey key{c->gen,          base_dt,       c->cfg.adaptive_dt, period,
+                                  c->b_cap_records, c->b_cap_list, c->groups_bound,    c->b_dmax,
+                                  c->b_grid_up,     c->b_grid_pair};
+      if (!c->gexec || !(key == c->gkey)) {
+        NBXCHK(capture(c, base_dt, period));
+        c->gkey = key;
+      }
+      launch_set_run(c->sc, s, c->stream);
+      for (int32_t r = 0; r < reps; ++r) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+      NBXCHK(sync_scalars(c));                            // the run's one wait: did every build fit?
+      if (c->hsc->abort_flags) {
+        ++c->tim.graph_aborts;
+        NBXCHK(restore(c, sn));
+        launch_set_run(c->sc, 0, c->stream);              // the flag would stop the walks of the steps below
+        first_dt();                                       // the speeds of the restored state
+      } else {
+        const int32_t done = reps * period;
+        s += done;
+        c->tim.steps += done;
+        c->tim.graph_steps += done;
+        c->tim.graph_launches += reps;
+        c->tim.graph_nodes = c->gnodes;
+        c->tim.graph_period = period;
+        c->total_nodes = (uint32_t)c->hsc->total_nodes;
+        c->max_depth = c->hsc->max_leaf_depth;
+        c->tim.nodes = c->total_nodes;
+        c->tim.depth = c->max_depth;
+        c->acc_valid = true;                              // the last walk's forces, at the current positions
+      }
+    }
   }
-  for (int32_t s = 0; s < nsteps; ++s) {
-    if (c->cfg.adaptive_dt) launch_dt(sy, base_dt, P<double>(c->dts), s, c->stream);
+  for (; s < nsteps; ++s) {
+    if (c->cfg.adaptive_dt) launch_dt(sy, base_dt, P<double>(c->dts), s, c->sc, c->stream);
     NBXCHK(step_device(c));
   }
   if (dts_out && c->cfg.adaptive_dt)
     HIPCHK(c, hipMemcpyAsync(dts_out, c->dts.p, (size_t)nsteps * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  ++c->tim.host_waits;
+  return NBX_OK;
+}
+
+nbx_status nbx_set_graph(nbx_ctx* c, int on) {
+  if (!c) return NBX_ERR_ARG;
+  if (on < 0 || on > 2) return fail(c, NBX_ERR_ARG, "graph mode must be 0, 1 or 2");
+  if (on != c->graph) {
+    c->graph = on;
+    drop_graph(c);
+    c->b_cap_records = c->b_cap_list = 0;
+    c->b_dmax = -1;
+  }
   return NBX_OK;
 }
 

@@ -189,6 +189,7 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   launch_bbox(St, A.at<BoxBlock>(kLayout.boxes), (int)nb, A.at<int32_t>(kLayout.box_off),
               A.at<double>(kLayout.box_part), sy, s);
   launch_fill_scalars(sc, 1 | 4 | 8, s);
+  launch_set_run(sc, 0, s);                                // no abort flag: the walk runs
   uint64_t* key[2] = {A.at<uint64_t>(kLayout.key[0]), A.at<uint64_t>(kLayout.key[1])};
   uint32_t* idx[2] = {A.at<uint32_t>(kLayout.idx[0]), A.at<uint32_t>(kLayout.idx[1])};
   uint64_t* key2[2] = {A.at<uint64_t>(kLayout.key2[0]), A.at<uint64_t>(kLayout.key2[1])};

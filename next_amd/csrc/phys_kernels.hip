@@ -118,6 +118,17 @@ __global__ __launch_bounds__(kBlock) void k_phys_pair(int64_t m, const int32_t* 
   pn[level_list[k]].skip = (int32_t)end_list[k];
 }
 
+// The same with the count on the device (fixed grid striding over it).
+__global__ __launch_bounds__(kBlock) void k_phys_pair_dev(const int32_t* __restrict__ level_list,
+                                                          const uint32_t* __restrict__ end_list,
+                                                          PhysNode* __restrict__ pn,
+                                                          const DevScalars* __restrict__ sc) {
+  if (sc->abort_flags) return;
+  const int64_t m = (int64_t)sc->level_off[kMaxDepthSlots - 1] + sc->level_count[kMaxDepthSlots - 1];
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += (int64_t)gridDim.x * kBlock)
+    pn[level_list[k]].skip = (int32_t)end_list[k];
+}
+
 // Internal nodes of one depth from their children (FP64), and the walk
 // groups this node packs: a node of more than G bodies packs its children of
 // at most G bodies into groups, consecutive siblings together while the sum
@@ -125,13 +136,9 @@ __global__ __launch_bounds__(kBlock) void k_phys_pair(int64_t m, const int32_t* 
 // Every body lands in exactly one group and a group's bodies lie inside one
 // node, so groups are compact in space where runs of G consecutive Morton
 // keys are not (such a run can straddle two distant cells).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg,
-                                                        int64_t cnt, bool root_level, PhysNode* __restrict__ pn,
-                                                        PhysAux* __restrict__ aux, double inv_theta, int G,
-                                                        int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= cnt) return;
-  const int32_t id = level_list[beg + k];
+__device__ __forceinline__ void phys_upward_node(int32_t id, bool root_level, PhysNode* __restrict__ pn,
+                                                 PhysAux* __restrict__ aux, double inv_theta, int G,
+                                                 int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
   const int32_t end = pn[id].skip;
   const int32_t first = pn[id].first;
   // One walk of the child chain (at most 8 children) records the child
@@ -243,6 +250,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
   aux[id] = a;
 }
 
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt, bool root_level,
+                   PhysNode* __restrict__ pn, PhysAux* __restrict__ aux, double inv_theta, int G,
+                   int32_t* __restrict__ gcnt, uint32_t* __restrict__ gmark) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= cnt) return;
+  phys_upward_node(level_list[beg + k], root_level, pn, aux, inv_theta, G, gcnt, gmark);
+}
+
+// Depth d of a build whose counts stay on the device (k_check): fixed grid
+// striding over the depth's extent in the node list.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void k_phys_upward_level(int d, const int32_t* __restrict__ level_list, PhysNode* __restrict__ pn,
+                         PhysAux* __restrict__ aux, double inv_theta, int G, int32_t* __restrict__ gcnt,
+                         uint32_t* __restrict__ gmark, const DevScalars* __restrict__ sc) {
+  if (sc->abort_flags) return;
+  const int64_t beg = sc->level_off[d], cnt = sc->level_count[d];
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * kBlock)
+    phys_upward_node(level_list[beg + k], d == 0, pn, aux, inv_theta, G, gcnt, gmark);
+}
+
 template <typename T>
 struct alignas(4 * sizeof(T)) V4 {
   T x, y, z, w;
@@ -300,7 +328,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   // XCDs round-robin: XCD x walks the contiguous (spatially coherent) range
   // [x per, (x + 1) per) of the groups with its gridDim / 8 blocks, so its L2
   // holds the part of the tree those groups touch, whatever the grid size.
-  const int64_t ngroups = (int64_t)sc->tree_groups;
+  const int64_t ngroups = sc->abort_flags ? 0 : (int64_t)sc->tree_groups;   // nothing after an aborted build
   const int64_t per = (ngroups + 7) / 8;
   const int64_t xbase = (int64_t)(blockIdx.x % 8) * per;
   for (int64_t j = blockIdx.x / 8; j < per; j += gridDim.x / 8) {
@@ -661,6 +689,17 @@ void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, co
 void launch_phys_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn, hipStream_t s) {
   if (m > 0)
     hipLaunchKernelGGL(k_phys_pair, dim3(grid_of(m, kBlock)), dim3(kBlock), 0, s, m, level_list, end_list, pn);
+}
+
+void launch_phys_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn,
+                          const DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_phys_pair_dev, dim3(grid), dim3(kBlock), 0, s, level_list, end_list, pn, sc);
+}
+
+void launch_phys_upward_level(unsigned grid, int d, const int32_t* level_list, PhysNode* pn, void* aux, double theta,
+                              int group_size, int32_t* gcnt, uint32_t* gmark, const DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_phys_upward_level, dim3(grid), dim3(kBlock), 0, s, d, level_list, pn,
+                     reinterpret_cast<PhysAux*>(aux), 1.0 / theta, group_size, gcnt, gmark, sc);
 }
 
 void launch_phys_upward(const int32_t* level_list, int64_t beg, int64_t cnt, int depth, PhysNode* pn, void* aux,

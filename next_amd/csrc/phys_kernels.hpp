@@ -55,6 +55,12 @@ void launch_phys_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, co
                       hipStream_t s);
 // Skip pointers of the m internal nodes from the two lists.
 void launch_phys_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn, hipStream_t s);
+// The same two passes for builds whose counts stay on the device (see
+// launch_check): fixed grids, counts from the scalars.
+void launch_phys_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn,
+                          const DevScalars* sc, hipStream_t s);
+void launch_phys_upward_level(unsigned grid, int d, const int32_t* level_list, PhysNode* pn, void* aux, double theta,
+                              int group_size, int32_t* gcnt, uint32_t* gmark, const DevScalars* sc, hipStream_t s);
 // Internal nodes of one depth from their children (FP64 accumulation), and
 // the walk groups of at most group_size bodies each node packs (see
 // k_phys_upward).  `aux`: kPhysAuxBytes per node of FP64 moments (mass,

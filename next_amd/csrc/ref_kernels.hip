@@ -65,6 +65,12 @@ __global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int wha
   if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; sc->n_eval = 0ull; }
 }
 
+// Start of a captured run: no abort yet, dt log entries from `step` on.
+__global__ void k_set_run(DevScalars* sc, int32_t step) {
+  sc->abort_flags = 0;
+  sc->step_count = step;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u32(int64_t n, uint32_t* __restrict__ a, uint32_t v) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) a[i] = v;
@@ -400,15 +406,40 @@ __global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const int32_t* __res
   nodes[level_list[k]].skip = (int32_t)end_list[k];
 }
 
+// The same with the count on the device (fixed grid striding over it).
+__global__ __launch_bounds__(kBlock) void k_pair_dev(const int32_t* __restrict__ level_list,
+                                                     const uint32_t* __restrict__ end_list,
+                                                     RefNode* __restrict__ nodes, const DevScalars* __restrict__ sc) {
+  if (sc->abort_flags) return;
+  const int64_t m = (int64_t)sc->level_off[kMaxDepthSlots - 1] + sc->level_count[kMaxDepthSlots - 1];
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += (int64_t)gridDim.x * kBlock)
+    nodes[level_list[k]].skip = (int32_t)end_list[k];
+}
+
+// A build whose counts stay on the device: each depth's offset in the node
+// list, and the abort flag if the tree does not fit the buffers and launches
+// sized beforehand (node records, node-list entries, depth levels), needs
+// the two-word keys, or has a non-finite position.  One wave.
+__global__ __launch_bounds__(kWave) void k_check(DevScalars* sc, int64_t cap_records, int64_t cap_list, int dmax) {
+  const int t = threadIdx.x;
+  const uint32_t v = (uint32_t)sc->level_count[t];
+  const uint32_t inc = wave_incl_scan_u32(v);
+  sc->level_off[t] = (int32_t)(inc - v);
+  if (t == kWave - 1) {
+    int32_t f = 0;
+    if ((int64_t)(uint32_t)sc->total_nodes + 1 > cap_records || (int64_t)inc > cap_list) f |= kAbortCapacity;
+    if (sc->max_leaf_depth - 1 > dmax) f |= kAbortDepth;
+    if (sc->collisions > 0) f |= kAbortCollision;
+    if (sc->err & 1) f |= kAbortNonFinite;
+    if (f) sc->abort_flags |= f;
+  }
+}
+
 // Mass, centre of mass and one-level softened quadrupole of the internal
 // nodes of one depth from their children (octree.h:104-126), and the
 // squared-distance thresholds of the walk's opening test for this theta.
-__global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt,
-                                                   RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
-                                                   double inv_theta) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= cnt) return;
-  const int32_t id = level_list[beg + k];
+__device__ __forceinline__ void upward_node(int32_t id, RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
+                                            double inv_theta) {
   const int32_t end = nodes[id].skip;
   const double size = nodes[id].size;
   // One walk of the child chain (at most 8; the bound also keeps a malformed
@@ -460,6 +491,26 @@ __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ l
   const double q2 = q * q;
   const float acc2 = (q2 > 1e-36 && q2 < 1e36) ? __double2float_ru(q2 * (1.0 + 2e-6)) : __builtin_nanf("");
   heads[id] = WalkHead{cx, cy, cz, acc2, M == 0.0 ? (end | kLinkZeroMass) : end};
+}
+
+__global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt,
+                                                   RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
+                                                   double inv_theta) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= cnt) return;
+  upward_node(level_list[beg + k], nodes, heads, inv_theta);
+}
+
+// The same for depth d of a build whose counts stay on the device: the
+// depth's extent in the node list comes from the scalars (k_check), the grid
+// is fixed and strides over it.
+__global__ __launch_bounds__(kBlock) void k_upward_level(int d, const int32_t* __restrict__ level_list,
+                                                         RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
+                                                         double inv_theta, const DevScalars* __restrict__ sc) {
+  if (sc->abort_flags) return;
+  const int64_t beg = sc->level_off[d], cnt = sc->level_count[d];
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * kBlock)
+    upward_node(level_list[beg + k], nodes, heads, inv_theta);
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks
@@ -584,7 +635,7 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
   // the node records come in through scalar loads into scalar registers.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int64_t gi = xcd_block(gridDim.x) * (kWalkBlock / kWave) + wave;
-  if (gi >= ngroups) return;                               // whole wave past the end
+  if (gi >= ngroups || sc->abort_flags) return;            // whole wave past the end, or an aborted build
   int32_t* const mine = &lists[wave][0][lane_id()];      // ring slot k at mine[k * kWave]
   const Group g = groups[gi];
   const int64_t i = g.first + lane_id();
@@ -801,7 +852,8 @@ __global__ __launch_bounds__(kBlock) void k_maxv2(int64_t n, StateArrays st, Sys
 
 // computeAdaptiveDt (adaptive.h:38-51) for each system from its max |v|^2;
 // dts (may be null) receives system 0's value for `step`.
-__global__ __launch_bounds__(kBlock) void k_dt(SysArrays sy, double base, double* dts, int64_t step) {
+__global__ __launch_bounds__(kBlock) void k_dt(SysArrays sy, double base, double* dts, int64_t step,
+                                               DevScalars* sc) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= sy.nsys) return;
   const double v2 = __longlong_as_double((long long)sy.v2max[s]);
@@ -816,7 +868,11 @@ __global__ __launch_bounds__(kBlock) void k_dt(SysArrays sy, double base, double
     if (base * 1.0 < dt) dt = base * 1.0;
   }
   sy.dt[s] = dt;
-  if (dts && s == 0) dts[step] = dt;
+  if (dts && s == 0) {
+    // step < 0: the entry comes from the device's step counter (captured runs)
+    const int64_t at = step >= 0 ? step : (int64_t)(sc->step_count++);
+    dts[at] = dt;
+  }
 }
 
 __device__ __forceinline__ uint32_t digit_at(const uint64_t* key, const uint64_t* key2, int32_t f, int l) {
@@ -935,6 +991,20 @@ void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const u
 void launch_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes, hipStream_t s) {
   if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, level_list, end_list, nodes);
 }
+void launch_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes,
+                     const DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_dev, dim3(grid), dim3(kBlock), 0, s, level_list, end_list, nodes, sc);
+}
+void launch_set_run(DevScalars* sc, int32_t step, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_run, dim3(1), dim3(1), 0, s, sc, step);
+}
+void launch_check(DevScalars* sc, int64_t cap_records, int64_t cap_list, int dmax, hipStream_t s) {
+  hipLaunchKernelGGL(k_check, dim3(1), dim3(kWave), 0, s, sc, cap_records, cap_list, dmax);
+}
+void launch_upward_level(unsigned grid, int d, const int32_t* level_list, RefNode* nodes, WalkHead* heads,
+                         double theta, const DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_upward_level, dim3(grid), dim3(kBlock), 0, s, d, level_list, nodes, heads, 1.0 / theta, sc);
+}
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
                    double theta, hipStream_t s) {
   if (cnt > 0)
@@ -969,8 +1039,8 @@ void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStre
     hipLaunchKernelGGL(k_maxv2, dim3((unsigned)div_up(n, (int64_t)kBlock * kMaxv2Items)), dim3(kBlock), 0, s, n, st,
                        sy);
 }
-void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, hipStream_t s) {
-  hipLaunchKernelGGL(k_dt, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, base, dts, step);
+void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_dt, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, base, dts, step, sc);
 }
 void launch_export(int64_t total, const RefNode* nodes, const uint64_t* key, const uint64_t* key2,
                    const int32_t* id, const SysArrays& sy, int32_t* depth, int32_t* leaf, int32_t* body,

@@ -63,6 +63,19 @@ void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const u
                  int32_t* level_list, uint32_t* end_list, uint32_t cap_nodes, uint32_t cap_int, hipStream_t s);
 // Skip pointers of the m internal nodes from the two lists.
 void launch_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes, hipStream_t s);
+// Builds whose counts stay on the device (no host read-back; a step of them
+// can be captured): k_check derives each depth's offset and raises the abort
+// flag when the tree does not fit cap_records node records, cap_list
+// node-list entries or depth levels 0..dmax; the pair and per-depth upward
+// kernels take their counts from the scalars with fixed grids, and every
+// tree kernel does nothing once the flag is set.
+void launch_check(DevScalars* sc, int64_t cap_records, int64_t cap_list, int dmax, hipStream_t s);
+// Start of a captured run: abort flag cleared, dt log entries from `step` on.
+void launch_set_run(DevScalars* sc, int32_t step, hipStream_t s);
+void launch_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes,
+                     const DevScalars* sc, hipStream_t s);
+void launch_upward_level(unsigned grid, int d, const int32_t* level_list, RefNode* nodes, WalkHead* heads,
+                         double theta, const DevScalars* sc, hipStream_t s);
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
                    double theta, hipStream_t s);
 void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s);
@@ -74,7 +87,9 @@ void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, con
 void launch_kick_drift(int64_t n, const StateArrays& st, const double* ax, const double* ay, const double* az,
                        const SysArrays& sy, float* amag, hipStream_t s);
 void launch_maxv2(int64_t n, const StateArrays& st, const SysArrays& sy, hipStream_t s);
-void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, hipStream_t s);
+// dts (may be null) receives system 0's dt at entry `step`, or, for step < 0,
+// at the entry the device's step counter gives (which it advances).
+void launch_dt(const SysArrays& sy, double base, double* dts, int64_t step, DevScalars* sc, hipStream_t s);
 void launch_export(int64_t total, const RefNode* nodes, const uint64_t* key, const uint64_t* key2,
                    const int32_t* id, const SysArrays& sy, int32_t* depth, int32_t* leaf, int32_t* body,
                    int32_t* cmask, double* geo4, double* mass, double* com3, double* q6, hipStream_t s);

@@ -31,7 +31,7 @@ EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", 
            "nbx_potential", "nbx_direct", "nbx_energy", "nbx_energy_systems", "nbx_upload_batch", "nbx_add_system",
            "nbx_download_system", "nbx_remove_system", "nbx_num_systems", "nbx_system_size", "nbx_server_create",
            "nbx_server_destroy", "nbx_server_submit", "nbx_server_poll", "nbx_server_wait", "nbx_server_stats",
-           "nbx_server_create_with_ops", "nbx_accel_noalloc"]
+           "nbx_server_create_with_ops", "nbx_accel_noalloc", "nbx_set_graph"]
 
 
 class NbxError(RuntimeError):
@@ -50,7 +50,9 @@ class Timing(C.Structure):
                 ("upward_ms", C.c_double), ("walk_ms", C.c_double), ("kick_ms", C.c_double),
                 ("total_ms", C.c_double), ("nodes", C.c_int64), ("depth", C.c_int32), ("n_pp", C.c_int64),
                 ("n_pc", C.c_int64), ("n_open", C.c_int64), ("n_iter", C.c_int64),
-                ("n_eval", C.c_int64)]
+                ("n_eval", C.c_int64), ("graph_steps", C.c_int64), ("graph_launches", C.c_int64),
+                ("graph_nodes", C.c_int64), ("graph_period", C.c_int64), ("graph_aborts", C.c_int64),
+                ("host_waits", C.c_int64)]
 
 
 class Request(C.Structure):
@@ -93,6 +95,7 @@ def lib():
     L.nbx_set_theta.argtypes = [VP, C.c_double]
     L.nbx_set_alpha.argtypes = [VP, C.c_double]
     L.nbx_set_rebuild_interval.argtypes = [VP, C.c_int32]
+    L.nbx_set_graph.argtypes = [VP, C.c_int]
     L.nbx_trim.argtypes = [VP]
     L.nbx_device_bytes.argtypes = [VP]
     L.nbx_device_bytes.restype = C.c_uint64
@@ -158,7 +161,7 @@ class Engine:
 
     def __init__(self, theta: float = 0.5, device: int = 0, mode: int = MODE_REFERENCE,
                  precision: int = PREC_FP64, eps: float = 0.01, leaf_size: int = 16, group_size: int = 64,
-                 adaptive_dt: bool = True, alpha: float = 0.0, rebuild_every: int = 1):
+                 adaptive_dt: bool = True, alpha: float = 0.0, rebuild_every: int = 1, graph: int = 0):
         L = lib()
         cfg = Config()
         L.nbx_config_default(C.byref(cfg))
@@ -175,6 +178,8 @@ class Engine:
             self.set_alpha(alpha)
         if rebuild_every != 1:
             self.set_rebuild_interval(rebuild_every)
+        if graph:
+            self.set_graph(graph)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -193,6 +198,11 @@ class Engine:
     def _check(self, st):
         if st != 0:
             raise NbxError(st, lib().nbx_last_error(self._h).decode())
+
+    def set_graph(self, on: int):
+        """nbx_set_graph: 1 replays nbx_run's steps from a captured HIP graph,
+        2 the same without headroom in the bounded builds, 0 step by step."""
+        self._check(lib().nbx_set_graph(self._h, int(on)))
 
     def set_theta(self, theta: float):
         self._check(lib().nbx_set_theta(self._h, theta))
