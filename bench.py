@@ -143,7 +143,30 @@ def cpu_baseline(p: ic.Particles, base_dt: float, theta: float, steps: int, c1: 
             "sample": sample, "host": host}
 
 
-def ensemble(nsys: int, steps: int, warmup: int) -> dict:
+def c1_plummer(steps: int = 100) -> dict:
+    """The Plummer configuration (N = 16,384, seed 1, virial ratio 0.5,
+    truncated at 10 a) in reference mode with NEXT's adaptive dt (base dt
+    1e-3): its 100 KDK steps after 4 warm-up steps, one by one and replayed
+    from a captured graph."""
+    from next_amd.nbx import Engine
+
+    p = ic.plummer(16384, seed=1)
+    ms = {}
+    for graph in (0, 1):
+        with Engine(theta=0.5, graph=graph) as e:
+            e.upload(p)
+            e.run(1e-3, 4)
+            e.reset_timing()
+            t0 = time.perf_counter()
+            e.run(1e-3, steps)
+            ms[graph] = (time.perf_counter() - t0) * 1e3 / steps
+            t = e.timing()
+    return {"workload": "Plummer N=16384 (seed 1), reference mode FP64, theta 0.5, base dt 1e-3, adaptive dt",
+            "steps": steps, "ms_per_step": round(ms[1], 4), "step_by_step_ms_per_step": round(ms[0], 4),
+            "host_waits_per_step": t["host_waits"] / steps}
+
+
+def ensemble(nsys: int, steps: int, warmup: int, ref_body_steps_per_s: float = None) -> dict:
     """Batched ensemble: nsys Plummer spheres (a = 1, M = 1, G = 1, eps = 0.01;
     samples of the distribution function, not rescaled to an exact virial
     ratio), N_i log-uniform in [1024, 65536] from seed i, theta = 0.5,
@@ -162,11 +185,25 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
         e.run(1e-3, warmup)
         t0 = time.perf_counter()
         e.run(1e-3, steps)
+        dt_eager = time.perf_counter() - t0
+        tim_eager = e.timing()
+        # the same steps replayed from a captured graph (nbx_set_graph)
+        e.set_graph(1)
+        e.run(1e-3, 2 * REBUILD_EVERY)                     # capture
+        e.reset_timing()
+        t0 = time.perf_counter()
+        e.run(1e-3, steps)
         dt = time.perf_counter() - t0
+        tg = e.timing()
     total = int(sum(sizes))
-    target = ensemble_target(systems, best)
+    target = ensemble_target(systems, best, ref_body_steps_per_s=ref_body_steps_per_s)
     return {"systems": nsys, "bodies": total, "steps": steps, "ensemble_steps_per_s": steps / dt,
             "ms_per_step": dt * 1e3 / steps, "body_steps_per_s": total * steps / dt,
+            "captured": {"graph_launches_per_step": tg["graph_launches"] / steps,
+                         "kernels_per_step": tg["graph_nodes"] / max(tg["graph_period"], 1),
+                         "host_waits_per_step": tg["host_waits"] / steps, "graph_aborts": tg["graph_aborts"],
+                         "steps_from_graph": tg["graph_steps"]},
+            "step_by_step_ms_per_step": dt_eager * 1e3 / steps,
             "autotuned": {"leaf_size": best[0], "group_size": best[1],
                           "trial_ms_per_step": {f"{lf}/{g}": round(float(m), 3) for (lf, g), m in zip(cands, trial_ms)}},
             "tree_rebuild_every": REBUILD_EVERY,
@@ -175,7 +212,8 @@ def ensemble(nsys: int, steps: int, warmup: int) -> dict:
             "at_target_accuracy": target}
 
 
-def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, energy_steps: int = 200) -> dict:
+def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, energy_steps: int = 200,
+                    ref_body_steps_per_s: float = None) -> dict:
     """The same ensemble with an opening criterion whose median force error
     against the FP64 direct sum is at most 1e-4 (relative criterion alpha
     behind a geometric guard theta): force error on 20,000 sampled bodies
@@ -216,6 +254,13 @@ def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, ener
             e.run(1e-3, 3)
             seq += (time.perf_counter() - t0) / 3
     log("plain FP64 path, systems one at a time: %.1f ms per ensemble step" % (seq * 1e3))
+    with Engine(theta=0.5) as e:       # the same, all 512 systems batched in one context
+        e.upload_batch(systems)
+        e.run(1e-3, 1)
+        t0 = time.perf_counter()
+        e.run(1e-3, 3)
+        batched = (time.perf_counter() - t0) / 3
+    log("plain FP64 path, systems batched: %.1f ms per ensemble step" % (batched * 1e3))
     return {"theta": theta, "alpha": alpha, "median_rel_force_err": float(np.median(err)),
             "p99_rel_force_err": float(np.percentile(err, 99)), "ms_per_step": round(ms, 3),
             "body_steps_per_s": total / (ms * 1e-3),
@@ -224,7 +269,13 @@ def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, ener
             "plain_fp64_one_system_at_a_time": {"ms_per_ensemble_step": round(seq * 1e3, 3),
                                                 "body_steps_per_s": total / seq,
                                                 "path": "reference mode (NEXT's FP64 algorithm), theta 0.5"},
-            "vs_plain_fp64_path": round(seq * 1e3 / ms, 2)}
+            "vs_plain_fp64_path": round(seq * 1e3 / ms, 2),
+            "plain_fp64_batched": {"ms_per_ensemble_step": round(batched * 1e3, 3), "body_steps_per_s": total / batched,
+                                   "path": "reference mode, the same systems in one context"},
+            "vs_plain_fp64_batched": round(batched * 1e3 / ms, 3),
+            # per-body step throughput against the single-system FP64 path on the 1M sphere (the headline)
+            "vs_plain_fp64_1M_single_system": (round(total / (ms * 1e-3) / ref_body_steps_per_s, 3)
+                                               if ref_body_steps_per_s else None)}
 
 
 def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, fp64_walk: bool = False) -> dict:
@@ -477,7 +528,7 @@ def main() -> int:
             res = physics(args.n, None, thetas=[float(t) for t in args.thetas.split(",")], energy_steps=0,
                           fp64_walk=args.walk_precision == "fp64")
         elif args.only == "ensemble":
-            res = ensemble(args.ensemble_systems, steps=8, warmup=1)
+            res = ensemble(args.ensemble_systems, steps=16, warmup=1)
         elif args.only == "merger":
             res = merger()
         else:
@@ -490,11 +541,25 @@ def main() -> int:
     eng.upload(p)
     eng.run(args.base_dt, args.warmup)
 
-    eng.set_profiling(1)      # HIP events around each phase, on the engine's stream
-    eng.reset_timing()
+    # the value: K steps one by one (nbx_run's default: each tree build reads
+    # its node counts back to the host while the layout kernel runs)
     t0 = time.perf_counter()
     eng.run(args.base_dt, args.steps)   # synchronous: returns after the stream drains
     wall_ms = (time.perf_counter() - t0) * 1e3
+    # beside it: K steps replayed from a captured graph (nbx_set_graph; bit for
+    # bit the same arithmetic, one host wait per run)
+    eng.set_graph(1)
+    eng.run(args.base_dt, 2)            # capture
+    eng.reset_timing()
+    t0 = time.perf_counter()
+    eng.run(args.base_dt, args.steps)
+    captured_ms = (time.perf_counter() - t0) * 1e3
+    tg = eng.timing()
+    eng.set_graph(0)
+
+    eng.set_profiling(1)      # HIP events around each phase, on the engine's stream (one by one)
+    eng.reset_timing()
+    eng.run(args.base_dt, args.steps)
     tim = eng.timing()
 
     eng.set_profiling(2)      # one more step with device-side interaction counters
@@ -547,13 +612,21 @@ def main() -> int:
                                                               / max(64 * cnt["n_eval"], 1), 4)},
         "phases_ms": {key: round(tim[key] / k, 4) for key in
                       ("kick_ms", "keys_ms", "sort_ms", "build_ms", "upward_ms", "walk_ms", "total_ms")},
+        "captured": {"ms_per_step": round(captured_ms / args.steps, 4), "step_by_step_ms_per_step": round(ms_per_step, 4),
+                     "graph_launches_per_step": tg["graph_launches"] / args.steps,
+                     "kernels_per_step": tg["graph_nodes"] / max(tg["graph_period"], 1),
+                     "host_waits_per_step": tg["host_waits"] / args.steps, "graph_aborts": tg["graph_aborts"],
+                     "steps_from_graph": tg["graph_steps"]},
         "tree": {"nodes": tim["nodes"], "depth": tim["depth"]},
         "phases_hbm": phase_rooflines(args.n, tim, k),
     }
-    log("headline: %.3f ms/step" % ms_per_step)
+    log("headline: %.3f ms/step (captured %.3f)" % (ms_per_step, captured_ms / args.steps))
+    out["c1"] = c1_plummer()
+    log("C1: %.3f ms/step captured, %.3f step by step" % (out["c1"]["ms_per_step"], out["c1"]["step_by_step_ms_per_step"]))
     if not args.no_extra:
         out["physics"] = physics(args.n, ms_per_step)
-        out["ensemble"] = ensemble(args.ensemble_systems, steps=8, warmup=1)
+        out["ensemble"] = ensemble(args.ensemble_systems, steps=16, warmup=1,
+                                   ref_body_steps_per_s=args.n / (ms_per_step * 1e-3))
         log("ensemble: %.1f ms/step" % out["ensemble"]["ms_per_step"])
         out["serving"] = []
         full = dict(n_max=1 << 20, steps=(50, 500)) if args.serve_config else {}

@@ -68,7 +68,7 @@ struct DevScalars {
 
 constexpr int32_t kAbortCapacity = 1;   // nodes or node-list entries beyond the buffers
 constexpr int32_t kAbortDepth = 2;      // deeper than the levels launched
-constexpr int32_t kAbortCollision = 4;  // bodies sharing 21 digits: the two-word key path is needed
+constexpr int32_t kAbortCollision = 4;  // bodies sharing all 42 digits (the eager build reports it)
 constexpr int32_t kAbortNonFinite = 8;  // a position is NaN or infinite
 
 // Per-system arrays: a context holds one or more independent particle

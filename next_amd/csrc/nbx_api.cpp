@@ -116,18 +116,19 @@ struct nbx_ctx {
     int adaptive, period;
     int64_t cap_records, cap_list, groups_bound;
     int dmax;
-    unsigned grid_up, grid_pair;
+    uint64_t grids;             // hash of the per-depth upward grids and the pair grid
     bool operator==(const GraphKey& o) const {
       return gen == o.gen && base_dt == o.base_dt && adaptive == o.adaptive && period == o.period &&
              cap_records == o.cap_records && cap_list == o.cap_list && groups_bound == o.groups_bound &&
-             dmax == o.dmax && grid_up == o.grid_up && grid_pair == o.grid_pair;
+             dmax == o.dmax && grids == o.grids;
     }
   } gkey{};
   int64_t gnodes = 0;         // kernels in the captured period
   // sizes of the bounded build (launch_check's limits and the fixed grids)
   int64_t b_cap_records = 0, b_cap_list = 0;
   int b_dmax = -1;
-  unsigned b_grid_up = 1, b_grid_pair = 1;
+  unsigned b_grid_up[kMaxDepthSlots] = {};   // per depth: blocks of the upward pass
+  unsigned b_grid_pair = 1;
   // the state a captured run starts from, restored if one of its builds aborts
   DBuf snap[kStateArrays], snap_acc[3], snap_amag;
 };
@@ -245,6 +246,7 @@ nbx_status ensure_particles(nbx_ctx* c, int64_t n, int64_t keep_n) {
     for (int w = 0; w < 2; ++w) {
       NBXCHK(ensure(c, c->key[w], (size_t)cap * 8));
       NBXCHK(ensure(c, c->idx[w], (size_t)cap * 4));
+      NBXCHK(ensure(c, c->key2[w], (size_t)cap * 8));
     }
     NBXCHK(ensure(c, c->lcp, (size_t)cap * 4));
     NBXCHK(ensure(c, c->cnt_s, (size_t)cap * 4));
@@ -349,6 +351,9 @@ const uint64_t* sort_bodies(nbx_ctx* c) {
     launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
     skey = P<uint64_t>(c->key[0]);
   }
+  // bodies sharing all digits of the first word: ordered by the second
+  launch_fix_runs(n, St, sy, skey, P<uint64_t>(c->key2[0]), c->amag_live ? P<uint32_t>(c->amag[c->acur]) : nullptr,
+                  s);
   record(c, EV_SORT);
   return skey;
 }
@@ -361,8 +366,8 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
-  launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
-                   c->sc, s);
+  launch_lcp_count(n, skey, P<uint64_t>(c->key2[0]), St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s),
+                   P<uint32_t>(c->lvl_part), c->sc, s);
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
   HIPCHK(c, hipMemcpyAsync(c->hsc, c->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipEventRecord(c->ev_scalars, s));
@@ -429,53 +434,11 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   ++c->tim.host_waits;
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
 
-  const uint64_t* skey2 = nullptr;
-  if (c->hsc->collisions > 0) {
-    c->spec_emit = false;                 // laid out from the first key word alone
-    // Bodies of one system sharing all 21 digits of the first key word: sort
-    // by (system, first word, second word), least significant first.
-    for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)c->cap * 8));
-    launch_keys2(n, St, sy, P<uint64_t>(c->key2[0]), s);
-    launch_iota(n, P<uint32_t>(c->idx[0]), s);
-    const bool a = radix_sort_u64(P<uint64_t>(c->key2[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key2[1]),
-                                  P<uint32_t>(c->idx[1]), n, 63, scr, s);
-    uint32_t* p1 = P<uint32_t>(c->idx[a ? 1 : 0]);
-    uint32_t* p1o = P<uint32_t>(c->idx[a ? 0 : 1]);
-    // first words of the current order, recomputed from the positions
-    launch_keys(n, St, sy, P<uint64_t>(c->key[0]), p1o, c->sc, s);
-    launch_gather_u64(n, p1, P<uint64_t>(c->key[0]), P<uint64_t>(c->key[1]), s);
-    const bool b = radix_sort_u64(P<uint64_t>(c->key[1]), p1, P<uint64_t>(c->key[0]), p1o, n, 63, scr, s);
-    const uint32_t* pf = b ? p1o : p1;
-    if (S > 1) {
-      uint32_t* pa = b ? p1o : p1;
-      uint32_t* pb = b ? p1 : p1o;
-      uint32_t* ka = P<uint32_t>(c->key2[0]);
-      uint32_t* kb = ka + n;
-      launch_gather_sys(n, pa, St.sys, ka, s);
-      const bool r = radix_sort_u32(ka, pa, kb, pb, n, sys_bits(S), scr, s);
-      pf = r ? pb : pa;
-    }
-    launch_gather_state(n, pf, St, view(c, 1 - c->cur), s);
-    c->cur = 1 - c->cur;
-    St = view(c, c->cur);
-    if (c->amag_live) {
-      launch_gather_u32(n, pf, P<uint32_t>(c->amag[c->acur]), P<uint32_t>(c->amag[1 - c->acur]), s);
-      c->acur = 1 - c->acur;
-    }
-    launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
-    launch_keys2(n, St, sy, P<uint64_t>(c->key2[0]), s);
-    skey = P<uint64_t>(c->key[0]);
-    skey2 = P<uint64_t>(c->key2[0]);
-    launch_fill_scalars(c->sc, 1, s);
-    launch_lcp_count(n, skey, skey2, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
-                     c->sc, s);
-    scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, tot_nodes, scr, s);
-    NBXCHK(sync_scalars(c));
-    if (c->hsc->coincident > 0)
-      return fail(c, NBX_ERR_DEPTH,
-                  std::to_string(c->hsc->coincident) +
-                      " pair(s) of bodies share all 42 octree digits (the tree is limited to 42 levels)");
-  }
+  if (c->hsc->coincident > 0)
+    return fail(c, NBX_ERR_DEPTH,
+                std::to_string(c->hsc->coincident) +
+                    " pair(s) of bodies share all 42 octree digits (the tree is limited to 42 levels)");
+  const uint64_t* skey2 = P<uint64_t>(c->key2[0]);
   c->skey = skey;
   c->skey2 = skey2;
   launch_sys_roots(sy, P<uint32_t>(c->cnt_s), c->sc, s);
@@ -686,11 +649,11 @@ int graph_period(const nbx_ctx* c) { return c->physics ? 2 * c->rebuild_every : 
 nbx_status prepare_bounded(nbx_ctx* c, int32_t nsteps) {
   const bool tight = c->graph == 2;
   const int64_t n = c->n, total = c->total_nodes, m = internal_count(*c->hsc);
-  int64_t maxlvl = 1;
-  for (int d = 0; d < kMaxDepthSlots; ++d) maxlvl = std::max<int64_t>(maxlvl, (uint32_t)c->hsc->level_count[d]);
   const int dneed = c->max_depth - 1;                      // deepest internal node of the last build
-  const bool keep = !tight && c->b_cap_records >= total + total / 16 + 64 && c->b_cap_list >= m + m / 16 + 64 &&
-                    c->b_dmax >= dneed + 1 && (int64_t)c->b_grid_up * kBlock >= maxlvl;
+  bool keep = !tight && c->b_cap_records >= total + total / 16 + 64 && c->b_cap_list >= m + m / 16 + 64 &&
+              c->b_dmax >= dneed;
+  for (int d = 0; keep && d <= dneed; ++d)
+    keep = (int64_t)c->b_grid_up[d] * kBlock >= (uint32_t)c->hsc->level_count[d];
   if (!keep) {
     const int64_t rec = tight ? total + 1 : total + total / 4 + 1024;
     const int64_t lst = tight ? std::max<int64_t>(m, 1) : m + m / 4 + 1024;
@@ -703,14 +666,21 @@ nbx_status prepare_bounded(nbx_ctx* c, int32_t nsteps) {
     }
     NBXCHK(ensure(c, c->level_list, (size_t)lst * 4));
     NBXCHK(ensure(c, c->end_list, (size_t)lst * 4));
-    c->b_dmax = tight ? dneed : std::min(dneed + 4, kMaxLevels);
-    const int64_t up = tight ? maxlvl : maxlvl + maxlvl / 4;
-    c->b_grid_up = (unsigned)std::min<int64_t>(std::max<int64_t>(div_up(up, kBlock), 1), 4096);
+    // depths 0 .. b_dmax launch one grid each; deeper ones run in the tail kernel
+    c->b_dmax = tight ? dneed : std::min(dneed + 1, kMaxLevels - 1);
+    // each depth's grid covers its count in the last build with an eighth to
+    // spare (the kernels stride over whatever count the device holds)
+    for (int d = 0; d < kMaxDepthSlots; ++d) {
+      const int64_t cnt = (uint32_t)c->hsc->level_count[d];
+      const int64_t want = tight ? cnt : cnt + cnt / 8 + 32;
+      c->b_grid_up[d] = (unsigned)std::min<int64_t>(std::max<int64_t>(div_up(want, kBlock), 1), 4096);
+    }
+    c->b_grid_pair = (unsigned)std::min<int64_t>(std::max<int64_t>(div_up(tight ? m : m + m / 8 + 64, kBlock), 1),
+                                                 8192);
   }
   c->b_cap_records = c->physics ? std::min<int64_t>(cap_words(c->pnodes, sizeof(PhysNode)), cap_words(c->paux, kPhysAuxBytes))
                                 : std::min<int64_t>(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead)));
   c->b_cap_list = cap_int(c);
-  c->b_grid_pair = (unsigned)std::min<int64_t>(std::max<int64_t>(div_up(c->b_cap_list, kBlock), 1), 8192);
   NBXCHK(ensure(c, c->scratch, std::max(sort_scratch_words(std::max(c->cap, c->b_cap_list)),
                                         scan_scratch_words(std::max(c->cap, c->b_cap_records))) * 4));
   NBXCHK(ensure(c, c->dts, (size_t)std::max(nsteps, 1) * 8));
@@ -736,11 +706,11 @@ void build_bounded(nbx_ctx* c) {
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
-  launch_lcp_count(n, skey, nullptr, St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part),
-                   c->sc, s);
+  launch_lcp_count(n, skey, P<uint64_t>(c->key2[0]), St.sys, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s),
+                   P<uint32_t>(c->lvl_part), c->sc, s);
   scan_u32(P<uint32_t>(c->cnt_s), P<uint32_t>(c->cnt_s), n, reinterpret_cast<uint32_t*>(&c->sc->total_nodes), scr,
            s);
-  launch_check(c->sc, c->b_cap_records, c->b_cap_list, c->b_dmax, s);
+  launch_check(c->sc, c->b_cap_records, c->b_cap_list, kMaxLevels, s);   // any depth: the tail kernel
   launch_sys_roots(sy, P<uint32_t>(c->cnt_s), c->sc, s);
   const uint32_t cap_rec = (uint32_t)c->b_cap_records, cap_lst = (uint32_t)c->b_cap_list;
   if (c->physics) {
@@ -751,8 +721,10 @@ void build_bounded(nbx_ctx* c) {
     launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, c->sc, pn,
                      P<int32_t>(c->level_list), P<uint32_t>(c->end_list), gcnt, gmark, cap_rec, cap_lst, s);
     launch_phys_pair_dev(c->b_grid_pair, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), pn, c->sc, s);
+    launch_phys_upward_tail(c->b_dmax + 1, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta, c->cfg.group_size,
+                            gcnt, gmark, c->sc, s);
     for (int d = c->b_dmax; d >= 0; --d)
-      launch_phys_upward_level(c->b_grid_up, d, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta,
+      launch_phys_upward_level(c->b_grid_up[d], d, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta,
                                c->cfg.group_size, gcnt, gmark, c->sc, s);
     launch_group_compact(n, gcnt, gmark, St.sys, P<Group>(c->tgroups), &c->sc->tree_groups, scr, s);
     c->ntgroups = -1;                                     // the walk's grid is groups_bound
@@ -762,12 +734,14 @@ void build_bounded(nbx_ctx* c) {
     launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, sy, c->sc, nodes,
                 P<WalkHead>(c->heads), P<int32_t>(c->level_list), P<uint32_t>(c->end_list), cap_rec, cap_lst, s);
     launch_pair_dev(c->b_grid_pair, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes, c->sc, s);
+    launch_upward_tail(c->b_dmax + 1, P<int32_t>(c->level_list), nodes, P<WalkHead>(c->heads), c->cfg.theta, c->sc,
+                       s);
     for (int d = c->b_dmax; d >= 0; --d)
-      launch_upward_level(c->b_grid_up, d, P<int32_t>(c->level_list), nodes, P<WalkHead>(c->heads), c->cfg.theta,
+      launch_upward_level(c->b_grid_up[d], d, P<int32_t>(c->level_list), nodes, P<WalkHead>(c->heads), c->cfg.theta,
                           c->sc, s);
   }
   c->skey = skey;
-  c->skey2 = nullptr;
+  c->skey2 = P<uint64_t>(c->key2[0]);
   c->spec_emit = false;
   c->tree_valid = true;
 }
@@ -783,8 +757,10 @@ void refit_bounded(nbx_ctx* c) {
   launch_phys_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), view(c, c->cur), c->sc, pn,
                    P<int32_t>(c->level_list), P<uint32_t>(c->end_list), gcnt, gmark, (uint32_t)c->b_cap_records, 0u,
                    s);
+  launch_phys_upward_tail(c->b_dmax + 1, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta, c->cfg.group_size,
+                          gcnt, gmark, c->sc, s);
   for (int d = c->b_dmax; d >= 0; --d)
-    launch_phys_upward_level(c->b_grid_up, d, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta,
+    launch_phys_upward_level(c->b_grid_up[d], d, P<int32_t>(c->level_list), pn, c->paux.p, c->cfg.theta,
                              c->cfg.group_size, gcnt, gmark, c->sc, s);
   c->tree_valid = true;
 }
@@ -1349,9 +1325,10 @@ nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) 
       NBXCHK(prepare_bounded(c, nsteps));
       Snapshot sn{};
       NBXCHK(snapshot(c, &sn));
-      const nbx_ctx::GraphKey key{c->gen,          base_dt,       c->cfg.adaptive_dt, period,
-                                  c->b_cap_records, c->b_cap_list, c->groups_bound,    c->b_dmax,
-                                  c->b_grid_up,     c->b_grid_pair};
+      uint64_t grids = 1469598103934665603ull ^ c->b_grid_pair;   // FNV-1a over the grids
+      for (unsigned g : c->b_grid_up) grids = (grids ^ g) * 1099511628211ull;
+      const nbx_ctx::GraphKey key{c->gen,        base_dt,         c->cfg.adaptive_dt, period, c->b_cap_records,
+                                  c->b_cap_list, c->groups_bound, c->b_dmax,          grids};
       if (!c->gexec || !(key == c->gkey)) {
         NBXCHK(capture(c, base_dt, period));
         c->gkey = key;

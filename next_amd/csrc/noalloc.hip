@@ -203,7 +203,9 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   uint32_t* cnt_s = A.at<uint32_t>(kLayout.cnt_s);
   uint32_t* part = A.at<uint32_t>(kLayout.part);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&sc->total_nodes);
-  launch_lcp_count(n, skey, nullptr, St.sys, lcp, cnt_s, part, sc, s);
+  // bodies sharing all digits of the first word: ordered by the second
+  launch_fix_runs(n, St, sy, skey, key2[0], nullptr, s);
+  launch_lcp_count(n, skey, key2[0], St.sys, lcp, cnt_s, part, sc, s);
   scan_u32(cnt_s, cnt_s, n, tot_nodes, scr, s);
   DevScalars h;
   {
@@ -211,32 +213,7 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
     if (r != NBX_OK) return r;
   }
   if (h.err & 1) return NBX_ERR_NONFINITE;
-  const uint64_t* skey2 = nullptr;
-  if (h.collisions > 0) {
-    // Bodies sharing all 21 digits of the first key word: sort by (first
-    // word, second word), least significant first, as the context path does.
-    launch_keys2(n, St, sy, key2[0], s);
-    launch_iota(n, idx[0], s);
-    const bool a = radix_sort_u64(key2[0], idx[0], key2[1], idx[1], n, 63, scr, s);
-    uint32_t* p1 = idx[a ? 1 : 0];
-    uint32_t* p1o = idx[a ? 0 : 1];
-    launch_keys(n, St, sy, key[0], p1o, sc, s);
-    launch_gather_u64(n, p1, key[0], key[1], s);
-    const bool b = radix_sort_u64(key[1], p1, key[0], p1o, n, 63, scr, s);
-    launch_gather_state(n, b ? p1o : p1, St, A.state(1 - cur), s);
-    cur = 1 - cur;
-    St = A.state(cur);
-    launch_keys(n, St, sy, key[0], idx[0], sc, s);
-    launch_keys2(n, St, sy, key2[0], s);
-    skey = key[0];
-    skey2 = key2[0];
-    launch_fill_scalars(sc, 1, s);
-    launch_lcp_count(n, skey, skey2, St.sys, lcp, cnt_s, part, sc, s);
-    scan_u32(cnt_s, cnt_s, n, tot_nodes, scr, s);
-    const nbx_status r = read_scalars(sc, &h, s);
-    if (r != NBX_OK) return r;
-    if (h.coincident > 0) return NBX_ERR_DEPTH;
-  }
+  if (h.coincident > 0) return NBX_ERR_DEPTH;
   const int64_t total = (uint32_t)h.total_nodes;
   const int64_t mi = internal_count(h);
   if (total > kNaNodes || mi > kNaNodes) return NBX_ERR_NOMEM;   // the tree does not fit the array

@@ -259,6 +259,22 @@ void k_phys_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t 
   phys_upward_node(level_list[beg + k], root_level, pn, aux, inv_theta, G, gcnt, gmark);
 }
 
+// Depths d_lo .. 41 of a build whose counts stay on the device in one
+// workgroup, deepest first, a barrier between depths (see k_upward_tail).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void k_phys_upward_tail(int d_lo, const int32_t* __restrict__ level_list, PhysNode* __restrict__ pn,
+                        PhysAux* __restrict__ aux, double inv_theta, int G, int32_t* __restrict__ gcnt,
+                        uint32_t* __restrict__ gmark, const DevScalars* __restrict__ sc) {
+  if (sc->abort_flags) return;
+  for (int d = kMaxLevels - 1; d >= d_lo; --d) {
+    const int64_t beg = sc->level_off[d], cnt = sc->level_count[d];
+    if (cnt == 0) continue;
+    for (int64_t k = threadIdx.x; k < cnt; k += kBlock)
+      phys_upward_node(level_list[beg + k], d == 0, pn, aux, inv_theta, G, gcnt, gmark);
+    __syncthreads();
+  }
+}
+
 // Depth d of a build whose counts stay on the device (k_check): fixed grid
 // striding over the depth's extent in the node list.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
@@ -694,6 +710,13 @@ void launch_phys_pair(int64_t m, const int32_t* level_list, const uint32_t* end_
 void launch_phys_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* end_list, PhysNode* pn,
                           const DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_phys_pair_dev, dim3(grid), dim3(kBlock), 0, s, level_list, end_list, pn, sc);
+}
+
+void launch_phys_upward_tail(int d_lo, const int32_t* level_list, PhysNode* pn, void* aux, double theta,
+                             int group_size, int32_t* gcnt, uint32_t* gmark, const DevScalars* sc, hipStream_t s) {
+  if (d_lo < kMaxLevels)
+    hipLaunchKernelGGL(k_phys_upward_tail, dim3(1), dim3(kBlock), 0, s, d_lo, level_list, pn,
+                       reinterpret_cast<PhysAux*>(aux), 1.0 / theta, group_size, gcnt, gmark, sc);
 }
 
 void launch_phys_upward_level(unsigned grid, int d, const int32_t* level_list, PhysNode* pn, void* aux, double theta,

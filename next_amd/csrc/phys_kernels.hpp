@@ -61,6 +61,8 @@ void launch_phys_pair_dev(unsigned grid, const int32_t* level_list, const uint32
                           const DevScalars* sc, hipStream_t s);
 void launch_phys_upward_level(unsigned grid, int d, const int32_t* level_list, PhysNode* pn, void* aux, double theta,
                               int group_size, int32_t* gcnt, uint32_t* gmark, const DevScalars* sc, hipStream_t s);
+void launch_phys_upward_tail(int d_lo, const int32_t* level_list, PhysNode* pn, void* aux, double theta,
+                             int group_size, int32_t* gcnt, uint32_t* gmark, const DevScalars* sc, hipStream_t s);
 // Internal nodes of one depth from their children (FP64 accumulation), and
 // the walk groups of at most group_size bodies each node packs (see
 // k_phys_upward).  `aux`: kPhysAuxBytes per node of FP64 moments (mass,

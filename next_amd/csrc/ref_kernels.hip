@@ -178,6 +178,44 @@ __global__ __launch_bounds__(kBlock) void k_keys2(int64_t n, StateArrays st, Sys
   key2[i] = lo;
 }
 
+// After the sort by the first key word, bodies of one system whose first
+// words are equal (they share all 21 digits) sit next to each other, in
+// their previous order.  Each such run is sorted in place by the second key
+// word (digits 21..41), stably: the order a sort by (system, first word,
+// second word) with ties in the previous order gives.  The second word is
+// written for the runs' bodies only, the only ones whose second word is
+// ever read (where two neighbours' first words are equal).
+__global__ __launch_bounds__(kBlock) void k_fix_runs(int64_t n, StateArrays st, SysArrays sy,
+                                                     const uint64_t* __restrict__ key, uint64_t* __restrict__ key2,
+                                                     uint32_t* __restrict__ amag) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t s = st.sys[i];
+  const uint64_t k = key[i];
+  const bool eq_next = i + 1 < n && st.sys[i + 1] == s && key[i + 1] == k;
+  const bool eq_prev = i > 0 && st.sys[i - 1] == s && key[i - 1] == k;
+  if (!eq_next || eq_prev) return;                           // not the first body of a run
+  int64_t j = i + 1;                                         // the run is [i, j]
+  while (j + 1 < n && st.sys[j + 1] == s && key[j + 1] == k) ++j;
+  double root[4];
+  sys_root(sy, s, root);
+  for (int64_t t = i; t <= j; ++t) {
+    uint64_t hi, lo;
+    key_digits(st.x[t], st.y[t], st.z[t], root, kKeyDigits, kMaxLevels, hi, lo);
+    key2[t] = lo;
+  }
+  for (int64_t t = i + 1; t <= j; ++t)
+    for (int64_t u = t; u > i && key2[u - 1] > key2[u]; --u) {
+      const int64_t v = u - 1;
+      auto sw_d = [&](double* a) { const double w = a[u]; a[u] = a[v]; a[v] = w; };
+      auto sw_i = [&](int32_t* a) { const int32_t w = a[u]; a[u] = a[v]; a[v] = w; };
+      sw_d(st.x); sw_d(st.y); sw_d(st.z); sw_d(st.vx); sw_d(st.vy); sw_d(st.vz); sw_d(st.m);
+      sw_i(st.type); sw_i(st.id);
+      const uint64_t w = key2[u]; key2[u] = key2[v]; key2[v] = w;
+      if (amag) { const uint32_t a = amag[u]; amag[u] = amag[v]; amag[v] = a; }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_keys_export(int64_t n, StateArrays st, SysArrays sy,
                                                         uint64_t* __restrict__ hi_out,
                                                         uint64_t* __restrict__ lo_out) {
@@ -429,7 +467,7 @@ __global__ __launch_bounds__(kWave) void k_check(DevScalars* sc, int64_t cap_rec
     int32_t f = 0;
     if ((int64_t)(uint32_t)sc->total_nodes + 1 > cap_records || (int64_t)inc > cap_list) f |= kAbortCapacity;
     if (sc->max_leaf_depth - 1 > dmax) f |= kAbortDepth;
-    if (sc->collisions > 0) f |= kAbortCollision;
+    if (sc->collisions > 0 || sc->coincident > 0) f |= kAbortCollision;
     if (sc->err & 1) f |= kAbortNonFinite;
     if (f) sc->abort_flags |= f;
   }
@@ -499,6 +537,23 @@ __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ l
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= cnt) return;
   upward_node(level_list[beg + k], nodes, heads, inv_theta);
+}
+
+// Depths d_lo .. 41 of a build whose counts stay on the device, deepest
+// first, in one workgroup (a barrier between depths: the waves share the
+// CU's cache, so a depth's results are visible to the next): the deep end
+// of a tree holds few nodes, chains mostly, and its depth varies from build
+// to build.
+__global__ __launch_bounds__(kBlock) void k_upward_tail(int d_lo, const int32_t* __restrict__ level_list,
+                                                        RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
+                                                        double inv_theta, const DevScalars* __restrict__ sc) {
+  if (sc->abort_flags) return;
+  for (int d = kMaxLevels - 1; d >= d_lo; --d) {
+    const int64_t beg = sc->level_off[d], cnt = sc->level_count[d];
+    if (cnt == 0) continue;
+    for (int64_t k = threadIdx.x; k < cnt; k += kBlock) upward_node(level_list[beg + k], nodes, heads, inv_theta);
+    __syncthreads();
+  }
 }
 
 // The same for depth d of a build whose counts stay on the device: the
@@ -944,6 +999,10 @@ void launch_keys_export(int64_t n, const StateArrays& st, const SysArrays& sy, u
                         hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_keys_export, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, hi, lo);
 }
+void launch_fix_runs(int64_t n, const StateArrays& st, const SysArrays& sy, const uint64_t* key, uint64_t* key2,
+                     uint32_t* amag, hipStream_t s) {
+  if (n > 1) hipLaunchKernelGGL(k_fix_runs, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, key, key2, amag);
+}
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(kBlock), 0, s, n, idx);
 }
@@ -1000,6 +1059,11 @@ void launch_set_run(DevScalars* sc, int32_t step, hipStream_t s) {
 }
 void launch_check(DevScalars* sc, int64_t cap_records, int64_t cap_list, int dmax, hipStream_t s) {
   hipLaunchKernelGGL(k_check, dim3(1), dim3(kWave), 0, s, sc, cap_records, cap_list, dmax);
+}
+void launch_upward_tail(int d_lo, const int32_t* level_list, RefNode* nodes, WalkHead* heads, double theta,
+                        const DevScalars* sc, hipStream_t s) {
+  if (d_lo < kMaxLevels)
+    hipLaunchKernelGGL(k_upward_tail, dim3(1), dim3(kBlock), 0, s, d_lo, level_list, nodes, heads, 1.0 / theta, sc);
 }
 void launch_upward_level(unsigned grid, int d, const int32_t* level_list, RefNode* nodes, WalkHead* heads,
                          double theta, const DevScalars* sc, hipStream_t s) {

@@ -36,6 +36,11 @@ void launch_bbox(const StateArrays& st, const BoxBlock* blocks, int nblocks, con
 void launch_keys(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint32_t* idx,
                  DevScalars* scw, hipStream_t s);
 void launch_keys2(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key2, hipStream_t s);
+// Runs of bodies of one system with equal first key words (in the order of
+// the sort by the first word) sorted in place by the second word, stably;
+// key2 written for those bodies (amag, may be null, moves with them).
+void launch_fix_runs(int64_t n, const StateArrays& st, const SysArrays& sy, const uint64_t* key, uint64_t* key2,
+                     uint32_t* amag, hipStream_t s);
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s);
 void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uint64_t* dst, hipStream_t s);
 void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint32_t* dst, hipStream_t s);
@@ -76,6 +81,9 @@ void launch_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* e
                      const DevScalars* sc, hipStream_t s);
 void launch_upward_level(unsigned grid, int d, const int32_t* level_list, RefNode* nodes, WalkHead* heads,
                          double theta, const DevScalars* sc, hipStream_t s);
+// Depths d_lo .. 41 (whatever the build holds there) in one workgroup.
+void launch_upward_tail(int d_lo, const int32_t* level_list, RefNode* nodes, WalkHead* heads, double theta,
+                        const DevScalars* sc, hipStream_t s);
 void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
                    double theta, hipStream_t s);
 void launch_fill_u32(int64_t n, uint32_t* a, uint32_t v, hipStream_t s);
