@@ -492,7 +492,8 @@ def main() -> int:
     ap.add_argument("--base-dt", type=float, default=5e-4)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the ensemble and serving samples")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C1, physics, ensemble and serving samples")
+    ap.add_argument("--no-graph", action="store_true", help="skip the captured (graph-replayed) run of the headline")
     ap.add_argument("--ensemble-systems", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--only", choices=["physics", "ensemble", "serving", "merger"],
@@ -548,14 +549,16 @@ def main() -> int:
     wall_ms = (time.perf_counter() - t0) * 1e3
     # beside it: K steps replayed from a captured graph (nbx_set_graph; bit for
     # bit the same arithmetic, one host wait per run)
-    eng.set_graph(1)
-    eng.run(args.base_dt, 2)            # capture
-    eng.reset_timing()
-    t0 = time.perf_counter()
-    eng.run(args.base_dt, args.steps)
-    captured_ms = (time.perf_counter() - t0) * 1e3
-    tg = eng.timing()
-    eng.set_graph(0)
+    captured_ms, tg = None, None
+    if not args.no_graph:
+        eng.set_graph(1)
+        eng.run(args.base_dt, 2)        # capture
+        eng.reset_timing()
+        t0 = time.perf_counter()
+        eng.run(args.base_dt, args.steps)
+        captured_ms = (time.perf_counter() - t0) * 1e3
+        tg = eng.timing()
+        eng.set_graph(0)
 
     eng.set_profiling(1)      # HIP events around each phase, on the engine's stream (one by one)
     eng.reset_timing()
@@ -612,18 +615,20 @@ def main() -> int:
                                                               / max(64 * cnt["n_eval"], 1), 4)},
         "phases_ms": {key: round(tim[key] / k, 4) for key in
                       ("kick_ms", "keys_ms", "sort_ms", "build_ms", "upward_ms", "walk_ms", "total_ms")},
-        "captured": {"ms_per_step": round(captured_ms / args.steps, 4), "step_by_step_ms_per_step": round(ms_per_step, 4),
-                     "graph_launches_per_step": tg["graph_launches"] / args.steps,
-                     "kernels_per_step": tg["graph_nodes"] / max(tg["graph_period"], 1),
-                     "host_waits_per_step": tg["host_waits"] / args.steps, "graph_aborts": tg["graph_aborts"],
-                     "steps_from_graph": tg["graph_steps"]},
+        "captured": ({"ms_per_step": round(captured_ms / args.steps, 4),
+                      "step_by_step_ms_per_step": round(ms_per_step, 4),
+                      "graph_launches_per_step": tg["graph_launches"] / args.steps,
+                      "kernels_per_step": tg["graph_nodes"] / max(tg["graph_period"], 1),
+                      "host_waits_per_step": tg["host_waits"] / args.steps, "graph_aborts": tg["graph_aborts"],
+                      "steps_from_graph": tg["graph_steps"]} if tg else None),
         "tree": {"nodes": tim["nodes"], "depth": tim["depth"]},
         "phases_hbm": phase_rooflines(args.n, tim, k),
     }
-    log("headline: %.3f ms/step (captured %.3f)" % (ms_per_step, captured_ms / args.steps))
-    out["c1"] = c1_plummer()
-    log("C1: %.3f ms/step captured, %.3f step by step" % (out["c1"]["ms_per_step"], out["c1"]["step_by_step_ms_per_step"]))
+    log("headline: %.3f ms/step (captured %s)" % (ms_per_step, "%.3f" % (captured_ms / args.steps) if tg else "-"))
     if not args.no_extra:
+        out["c1"] = c1_plummer()
+        log("C1: %.3f ms/step captured, %.3f step by step" % (out["c1"]["ms_per_step"],
+                                                            out["c1"]["step_by_step_ms_per_step"]))
         out["physics"] = physics(args.n, ms_per_step)
         out["ensemble"] = ensemble(args.ensemble_systems, steps=16, warmup=1,
                                    ref_body_steps_per_s=args.n / (ms_per_step * 1e-3))
