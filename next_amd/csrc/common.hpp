@@ -46,7 +46,7 @@ static_assert(sizeof(WalkHead) == 32, "WalkHead layout");
 // Small per-context device block: flags and the per-level node counts the
 // host needs to launch the upward pass.
 struct DevScalars {
-  int32_t err;              // bit 0: non-finite position
+  int32_t err;              // bit 0: non-finite position; bit 1: radix sort wait ran out
   int32_t collisions;       // adjacent equal primary keys
   int32_t coincident;       // adjacent equal primary and secondary keys
   int32_t total_nodes;              // (internal nodes: the sum of level_count)
@@ -70,6 +70,7 @@ constexpr int32_t kAbortCapacity = 1;   // nodes or node-list entries beyond the
 constexpr int32_t kAbortDepth = 2;      // deeper than the levels launched
 constexpr int32_t kAbortCollision = 4;  // bodies sharing all 42 digits (the eager build reports it)
 constexpr int32_t kAbortNonFinite = 8;  // a position is NaN or infinite
+constexpr int32_t kAbortSort = 16;      // the radix sort's wait for earlier tiles ran out
 
 // Per-system arrays: a context holds one or more independent particle
 // systems, stored one after another in key order.

@@ -11,9 +11,10 @@
 // pairs every node with its skip target, and the node list is sorted by
 // depth and, within a depth, by preorder: the upward pass's order.
 //
-// The count of earlier bodies comes from the scanned per-block rows of
-// k_lcp_count (columns 0..63 starts, 64..127 ends, per depth) plus a rank
-// inside the block from wave ballots, in body order.
+// The count of earlier bodies comes from the scanned per-block counts of
+// k_lcp_count (columns 0..63 starts, 64..127 ends, per depth; column c of
+// block b at part[c * blocks + b], with the same blocks as k_lcp_count) plus
+// a rank inside the block from wave ballots, in body order.
 #pragma once
 
 #include "common.hpp"
@@ -24,10 +25,11 @@ namespace nbx {
 // Called by every thread of a kBlock-thread block (valid: the thread holds
 // a body).  lp, li: lcp of the body with its predecessor and successor;
 // pre: preorder index of its first started node; skipv: the preorder index
-// after the nodes it ends.  part_row: this block's row of scanned counts.
+// after the nodes it ends.  part: the scanned counts (this block's at
+// part[c * gridDim.x + blockIdx.x]).
 // Entries at or beyond cap are not written.
 __device__ __forceinline__ void place_internal_nodes(bool valid, int lp, int li, uint32_t pre, uint32_t skipv,
-                                                     const uint32_t* __restrict__ part_row,
+                                                     const uint32_t* __restrict__ part,
                                                      const DevScalars* __restrict__ sc,
                                                      int32_t* __restrict__ level_list,
                                                      uint32_t* __restrict__ end_list, uint32_t cap) {
@@ -86,11 +88,13 @@ __device__ __forceinline__ void place_internal_nodes(bool valid, int lp, int li,
     const bool fe = valid && li < d && d <= lp;
     const uint64_t ms = __builtin_amdgcn_ballot_w64(fs), me = __builtin_amdgcn_ballot_w64(fe);
     if (fs) {
-      const uint32_t pos = off[d] + part_row[d] + wc[0][w][d] + (uint32_t)__popcll(ms & lt);
+      const uint32_t pos = off[d] + part[(int64_t)d * gridDim.x + blockIdx.x] + wc[0][w][d] +
+                           (uint32_t)__popcll(ms & lt);
       if (pos < cap) level_list[pos] = (int32_t)(pre + (uint32_t)(d - lp - 1));
     }
     if (fe) {
-      const uint32_t pos = off[d] + part_row[kMaxDepthSlots + d] + wc[1][w][d] + (uint32_t)__popcll(me & lt);
+      const uint32_t pos = off[d] + part[(int64_t)(kMaxDepthSlots + d) * gridDim.x + blockIdx.x] + wc[1][w][d] +
+                           (uint32_t)__popcll(me & lt);
       if (pos < cap) end_list[pos] = skipv;
     }
   }

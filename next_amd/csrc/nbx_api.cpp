@@ -301,6 +301,7 @@ nbx_status set_systems(nbx_ctx* c) {
   c->acc_valid = false;
   c->tree_valid = false;
   c->layout_valid = false;
+  c->max_depth = 0;           // the next build sorts every digit
   return NBX_OK;
 }
 
@@ -310,12 +311,14 @@ int sys_bits(int nsys) {
   return b;
 }
 
-// First half of a tree build (launches only): bounding boxes, keys, sort,
-// gather of the state into key order (flips c->cur), lcp, node counts and
-// their scan, and the copy of the device scalars to the host mirror.
 // Bounding boxes, keys, sort by key and then system, gather of the state
-// into key order (flips c->cur): the keys of the new order.
-const uint64_t* sort_bodies(nbx_ctx* c) {
+// into key order (flips c->cur): the keys of the new order.  The sort covers
+// the first `digits` octree digits of the keys (the depth of a recent build:
+// bodies in different leaves of such a tree differ there; 0: all 21 digits of
+// the first word), rounded up to whole 8-bit passes; k_fix_runs then orders
+// the runs of bodies that agree on those bits by their whole keys, so the
+// order is the full keys' whatever the depth of this build.
+const uint64_t* sort_bodies(nbx_ctx* c, int digits) {
   const int64_t n = c->n;
   const int S = (int)c->sys_n.size();
   hipStream_t s = c->stream;
@@ -326,9 +329,11 @@ const uint64_t* sort_bodies(nbx_ctx* c) {
   launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
   record(c, EV_KEYS);
   uint32_t* scr = P<uint32_t>(c->scratch);
+  const int passes = digits > 0 && digits < kKeyDigits ? (3 * digits + 7) / 8 : 8;
+  const int lo = std::max(0, 63 - 8 * passes);
   const bool in1 = radix_sort_u64(P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key[1]),
-                                  P<uint32_t>(c->idx[1]), n, 63, scr, s);
-  const uint64_t* skey = P<uint64_t>(c->key[in1 ? 1 : 0]);
+                                  P<uint32_t>(c->idx[1]), n, lo, 63 - lo, scr, &c->sc->err, s);
+  uint64_t* skey = P<uint64_t>(c->key[in1 ? 1 : 0]);
   const uint32_t* perm = P<uint32_t>(c->idx[in1 ? 1 : 0]);
   if (S > 1) {
     // stable pass by system: system-major, key order within each system
@@ -337,7 +342,7 @@ const uint64_t* sort_bodies(nbx_ctx* c) {
     uint32_t* ka = P<uint32_t>(c->key[in1 ? 0 : 1]);       // the free key buffer, as 2n words
     uint32_t* kb = ka + n;
     launch_gather_sys(n, pa, St.sys, ka, s);
-    const bool r = radix_sort_u32(ka, pa, kb, pb, n, sys_bits(S), scr, s);
+    const bool r = radix_sort_u32(ka, pa, kb, pb, n, 0, sys_bits(S), scr, &c->sc->err, s);
     perm = r ? pb : pa;
   }
   launch_gather_state(n, perm, St, view(c, 1 - c->cur), s);
@@ -351,9 +356,9 @@ const uint64_t* sort_bodies(nbx_ctx* c) {
     launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
     skey = P<uint64_t>(c->key[0]);
   }
-  // bodies sharing all digits of the first word: ordered by the second
+  // bodies agreeing on the sorted bits: ordered by their whole keys
   launch_fix_runs(n, St, sy, skey, P<uint64_t>(c->key2[0]), c->amag_live ? P<uint32_t>(c->amag[c->acur]) : nullptr,
-                  s);
+                  lo, s);
   record(c, EV_SORT);
   return skey;
 }
@@ -361,7 +366,7 @@ const uint64_t* sort_bodies(nbx_ctx* c) {
 nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   const int64_t n = c->n;
   hipStream_t s = c->stream;
-  const uint64_t* skey = sort_bodies(c);
+  const uint64_t* skey = sort_bodies(c, c->max_depth);
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
@@ -433,6 +438,8 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   HIPCHK(c, hipEventSynchronize(c->ev_scalars));
   ++c->tim.host_waits;
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
+  if (c->hsc->err & kSortErrTimeout)
+    return fail(c, NBX_ERR_HIP, "radix sort: a tile's wait for the tiles before it ran out");
 
   if (c->hsc->coincident > 0)
     return fail(c, NBX_ERR_DEPTH,
@@ -702,7 +709,7 @@ nbx_status prepare_bounded(nbx_ctx* c, int32_t nsteps) {
 void build_bounded(nbx_ctx* c) {
   const int64_t n = c->n;
   hipStream_t s = c->stream;
-  const uint64_t* skey = sort_bodies(c);
+  const uint64_t* skey = sort_bodies(c, c->max_depth);   // the depth when the run was captured
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);

@@ -61,7 +61,7 @@ constexpr Layout make_layout() {
   L.cnt_s = take((size_t)kNaBodies * 4);
   L.part = take((size_t)(((kNaBodies + kBlock - 1) / kBlock) * kPartW) * 4);
   const int64_t big = kNaBodies > kNaNodes ? kNaBodies : kNaNodes;
-  const size_t sw = (size_t)(kSortBins * ((big + kSortTile - 1) / kSortTile) + kSortBins);
+  const size_t sw = sort_scratch_words(big);
   const size_t cw = (size_t)((big + kScanTile - 1) / kScanTile + 2);
   L.sortscr = take((sw > cw ? sw : cw) * 4);
   L.nodes = take((size_t)(kNaNodes + 1) * sizeof(RefNode));   // one spare record for the walk
@@ -194,17 +194,17 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   uint32_t* idx[2] = {A.at<uint32_t>(kLayout.idx[0]), A.at<uint32_t>(kLayout.idx[1])};
   uint64_t* key2[2] = {A.at<uint64_t>(kLayout.key2[0]), A.at<uint64_t>(kLayout.key2[1])};
   launch_keys(n, St, sy, key[0], idx[0], sc, s);
-  const bool in1 = radix_sort_u64(key[0], idx[0], key[1], idx[1], n, 63, scr, s);
+  const bool in1 = radix_sort_u64(key[0], idx[0], key[1], idx[1], n, 0, 63, scr, &sc->err, s);
   launch_gather_state(n, idx[in1 ? 1 : 0], St, A.state(1 - cur), s);
   cur = 1 - cur;
   St = A.state(cur);
-  const uint64_t* skey = key[in1 ? 1 : 0];
+  uint64_t* skey = key[in1 ? 1 : 0];
   int32_t* lcp = A.at<int32_t>(kLayout.lcp);
   uint32_t* cnt_s = A.at<uint32_t>(kLayout.cnt_s);
   uint32_t* part = A.at<uint32_t>(kLayout.part);
   uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&sc->total_nodes);
   // bodies sharing all digits of the first word: ordered by the second
-  launch_fix_runs(n, St, sy, skey, key2[0], nullptr, s);
+  launch_fix_runs(n, St, sy, skey, key2[0], nullptr, 0, s);
   launch_lcp_count(n, skey, key2[0], St.sys, lcp, cnt_s, part, sc, s);
   scan_u32(cnt_s, cnt_s, n, tot_nodes, scr, s);
   DevScalars h;
@@ -213,6 +213,7 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
     if (r != NBX_OK) return r;
   }
   if (h.err & 1) return NBX_ERR_NONFINITE;
+  if (h.err & kSortErrTimeout) return NBX_ERR_HIP;
   if (h.coincident > 0) return NBX_ERR_DEPTH;
   const int64_t total = (uint32_t)h.total_nodes;
   const int64_t mi = internal_count(h);

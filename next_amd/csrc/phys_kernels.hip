@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_phys_emit(int64_t n, const int32_t* 
     pre = pre_off[i];
     skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
   }
-  place_internal_nodes(valid, lp, li, pre, skipv, part + (int64_t)blockIdx.x * kPartW, sc, level_list, end_list,
+  place_internal_nodes(valid, lp, li, pre, skipv, part, sc, level_list, end_list,
                        cap_int);
   if (!valid) return;
   const int ns = li > lp ? li - lp : 0;

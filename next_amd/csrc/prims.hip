@@ -4,13 +4,26 @@
 // sum of the preceding tile sums, which every tile forms itself).  No
 // inter-workgroup waiting.
 //
-// Radix sort: 8-bit digits, three launches per digit (tile histograms, one
-// workgroup per digit scanning that digit's column over the tiles, scatter;
-// the scatter adds the exclusive prefix of the digit totals, which every tile
-// forms itself from 256 values).  The scatter ranks keys
-// stably inside a tile with wave64 ballots (8 ballots give each lane the set
-// of lanes holding the same digit), stages the tile in LDS in digit order and
-// writes it out in runs, so consecutive lanes store consecutive addresses.
+// Radix sort: 8-bit digits, one launch per digit after one launch that
+// counts every digit of every key (the counts of each digit position are all
+// a pass needs to place its digit groups).  A pass takes tiles in the order of
+// a ticket counter, ranks its keys stably inside the tile with wave64 ballots
+// (8 ballots give each lane the set of lanes holding the same digit), and
+// learns how many keys of each digit the tiles before it hold by a two-level
+// decoupled look-back: each tile publishes its own digit counts at once; the
+// last tile of each group of kSortGroup tiles adds up its group and publishes
+// the group's counts, then the running totals up to its group's end when it
+// has them, one word per (tile or group, digit) with the kind in the top two
+// bits.  A tile adds the words of the tiles before it in its group and walks
+// back over the groups before its own until it meets a running total, so it
+// reads a few dozen words per digit however many tiles run at once.  A tile
+// only waits for tiles holding earlier tickets, which are resident and wait on
+// nobody later, so the waits end; they are still bounded, and a wait that runs
+// out sets an error bit instead of hanging.  The tile is staged in LDS in
+// digit order and written out in runs, so consecutive lanes store consecutive
+// addresses.
+#include <algorithm>
+
 #include "common.hpp"
 #include "prims.hpp"
 
@@ -64,52 +77,73 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* in, ui
   }
 }
 
-template <typename K>
-__global__ __launch_bounds__(kSortBlock) void k_radix_hist(const K* __restrict__ keys, int64_t n, int shift,
-                                                           uint32_t* __restrict__ hist, int64_t ntiles) {
-  __shared__ uint32_t h[kSortBlock / kWave][kSortBins];
-  uint32_t* hf = &h[0][0];
-  for (int i = threadIdx.x; i < (kSortBlock / kWave) * kSortBins; i += kSortBlock) hf[i] = 0;
-  __syncthreads();
-  const int w = threadIdx.x / kWave;
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
-#pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
-    int64_t i = base + j * kSortBlock + threadIdx.x;
-    if (i < n) atomicAdd(&h[w][(uint32_t)(keys[i] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  uint32_t c = 0;
-#pragma unroll
-  for (int k = 0; k < kSortBlock / kWave; ++k) c += h[k][threadIdx.x];
-  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = c;
+constexpr uint32_t kLbAggregate = 1u << 30;   // the tile's (group's) own count
+constexpr uint32_t kLbPrefix = 2u << 30;      // count of this group and every earlier one
+constexpr uint32_t kLbKind = 3u << 30;
+constexpr uint32_t kLbCount = (1u << 30) - 1;
+constexpr uint32_t kLbSpinLimit = 1u << 20;   // sleeps per lane per look-back, far above any real wait
+constexpr int kLbWindow = 16;                 // group words loaded per round of the group look-back
+constexpr int kHistRounds = 8;                // keys per thread per round of the counting launch
+constexpr int kHistStride = kSortMaxPasses * kSortBins;   // words between two count slots
+
+__global__ __launch_bounds__(kScanBlock) void k_sort_clear(uint32_t* __restrict__ w, int64_t nw) {
+  for (int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x; i < nw; i += (int64_t)gridDim.x * kScanBlock)
+    w[i] = 0u;
 }
 
-// Digit d's column of the tile histograms (hist[d * ntiles + t]) becomes its
-// exclusive prefix over the tiles; coltot[d] receives the column total.
-__global__ __launch_bounds__(kSortBlock) void k_radix_colscan(uint32_t* __restrict__ hist, int64_t ntiles,
-                                                              uint32_t* __restrict__ coltot) {
-  __shared__ uint32_t scr[kSortBlock / kWave + 1];
-  uint32_t* col = hist + (int64_t)blockIdx.x * ntiles;
-  uint32_t carry = 0;
-  for (int64_t c = 0; c < ntiles; c += kSortBlock) {
-    const int64_t i = c + threadIdx.x;
-    const uint32_t v = i < ntiles ? col[i] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan_u32<kSortBlock>(v, scr, &tot);
-    if (i < ntiles) col[i] = carry + ex;
-    carry += tot;
+// Counts of every digit position (hist[p * 256 + d]), kHistRounds keys per
+// thread per round of loads.  Keys arrive nearly sorted (the bodies' last order), so a
+// wave's high digits are mostly all alike: such a digit is added once for
+// the wave instead of once per lane.
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int lo, int passes,
+                                                          uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kSortMaxPasses * kSortBins];
+  for (int i = threadIdx.x; i < kSortMaxPasses * kSortBins; i += kSortBlock) h[i] = 0;
+  __syncthreads();
+  const int l = threadIdx.x % kWave;
+  const int64_t span = (int64_t)kSortBlock * kHistRounds;
+  for (int64_t base = (int64_t)blockIdx.x * span; base < n; base += (int64_t)gridDim.x * span) {
+    K kr[kHistRounds];                        // all loads in flight before the counting
+#pragma unroll
+    for (int r = 0; r < kHistRounds; ++r) {
+      const int64_t i = base + (int64_t)r * kSortBlock + threadIdx.x;
+      kr[r] = i < n ? keys[i] : K(0);
+    }
+#pragma unroll
+    for (int r = 0; r < kHistRounds; ++r) {
+      const bool valid = base + (int64_t)r * kSortBlock + threadIdx.x < n;   // invalid lanes: a wave's last ones
+      const uint64_t act = __ballot(valid);
+      for (int p = 0; p < passes; ++p) {
+        const uint32_t d = (uint32_t)(kr[r] >> (lo + 8 * p)) & 255u;
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__ballot(valid && d == d0) == act) {
+          if (l == 0 && act) atomicAdd(&h[p * kSortBins + d0], (uint32_t)__popcll(act));
+        } else if (valid) {
+          atomicAdd(&h[p * kSortBins + d], 1u);
+        }
+      }
+    }
   }
-  if (threadIdx.x == 0) coltot[blockIdx.x] = carry;
+  __syncthreads();
+  uint32_t* slot = hist + (blockIdx.x % kSortHistSlots) * kHistStride;
+  for (int i = threadIdx.x; i < passes * kSortBins; i += kSortBlock)
+    if (h[i]) atomicAdd(&slot[i], h[i]);
 }
 
+// One digit pass: the tile with ticket `tile` moves keys [tile * kSortTile,
+// +kSortTile) to their places.  hist: this digit's count slots over all keys
+// (slot k at hist + k * kHistStride); ticket: this pass's tile counter;
+// tw, gw: this pass's words per (tile, digit) and (group, digit); ticket, tw
+// and gw zero before the launch.  Grid: exactly ntiles = div_up(n, kSortTile)
+// workgroups.
 template <typename K>
-__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restrict__ kin,
-                                                              const uint32_t* __restrict__ vin,
-                                                              K* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                              int64_t n, int shift,
-                                                              const uint32_t* __restrict__ off, int64_t ntiles,
-                                                              const uint32_t* __restrict__ coltot) {
+__global__ __launch_bounds__(kSortBlock) void k_sort_pass(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                          K* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                          int64_t n, int64_t ntiles, int shift,
+                                                          const uint32_t* __restrict__ hist,
+                                                          uint32_t* __restrict__ ticket, uint32_t* __restrict__ tw,
+                                                          uint32_t* __restrict__ gw, int32_t* __restrict__ err) {
   constexpr int W = kSortBlock / kWave;
   __shared__ K sk[kSortTile];
   __shared__ uint32_t sv[kSortTile];
@@ -117,14 +151,21 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
   __shared__ uint32_t dstart[kSortBins];
   __shared__ uint32_t gbase[kSortBins];
   __shared__ uint32_t scr[W + 1];
+  __shared__ uint32_t hc[kSortBins];
+  __shared__ uint32_t tile_s;
   const int t = threadIdx.x, w = t / kWave, l = t % kWave;
   uint32_t* wcf = &wc[0][0];
   for (int i = t; i < W * kSortBins; i += kSortBlock) wcf[i] = 0;
+  if (t < kSortBins) hc[t] = 0;
+  if (t == 0) tile_s = atomicAdd(ticket, 1u);
   __syncthreads();
+  const int64_t tile = tile_s;
+  const int64_t g = tile / kSortGroup, g0 = g * kSortGroup;
+  const bool last = tile - g0 == kSortGroup - 1 || tile == ntiles - 1;   // publishes its group's count
 
   // Wave w ranks elements [tile + w*64*ITEMS, +64*ITEMS) in order: item j of
   // lane l is element j*64 + l, so per-wave counters see them in input order.
-  const int64_t tbase = (int64_t)blockIdx.x * kSortTile;
+  const int64_t tbase = tile * kSortTile;
   const int64_t wbase = tbase + (int64_t)w * kWave * kSortItems;
   const uint64_t ltmask = lanemask_lt();
   K k[kSortItems];
@@ -135,6 +176,64 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
     const bool valid = idx < n;
     k[j] = valid ? kin[idx] : K(0);
     v[j] = valid ? vin[idx] : 0u;
+  }
+  // The tile's digit counts first (a wave adds a digit all its lanes share
+  // once), published before the ranking so that later tiles find them.
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const bool valid = wbase + j * kWave + l < n;   // the invalid lanes of a wave are its last ones
+    const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t act = __ballot(valid);
+    if (__ballot(valid && d == d0) == act) {
+      if (l == 0 && act) atomicAdd(&hc[d0], (uint32_t)__popcll(act));
+    } else if (valid) {
+      atomicAdd(&hc[d], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t spins = 0;
+  bool out = false;   // a wait ran out
+  auto wait = [&]() {
+    if (++spins > kLbSpinLimit) {
+      if (err) atomicOr(err, kSortErrTimeout);
+      out = true;
+    } else {
+      __builtin_amdgcn_s_sleep(1);
+    }
+  };
+  // keys of digit t in the earlier tiles of the group (all loaded at once)
+  uint32_t ip = 0;
+  auto in_group = [&]() {
+    for (int64_t q = g0; q < tile && !out;) {
+      uint32_t wv[kSortGroup - 1];
+#pragma unroll
+      for (int j = 0; j < kSortGroup - 1; ++j)
+        wv[j] = q + j < tile ? __hip_atomic_load(&tw[(q + j) * kSortBins + t], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0u;
+      int j = 0;
+      for (; j < kSortGroup - 1 && q + j < tile; ++j) {
+        if ((wv[j] & kLbKind) == 0u) break;   // not published yet: reload from here
+        ip += wv[j] & kLbCount;
+      }
+      q += j;
+      if (q < tile) wait();
+    }
+  };
+  if (t < kSortBins) {
+    __hip_atomic_store(&tw[tile * kSortBins + t], kLbAggregate | hc[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (last) {   // the group's count, as early as the group's tiles allow
+      in_group();
+      __hip_atomic_store(&gw[g * kSortBins + t], (g == 0 ? kLbPrefix : kLbAggregate) | (ip + hc[t]),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t idx = wbase + j * kWave + l;
+    const bool valid = idx < n;
     const uint32_t d = (uint32_t)(k[j] >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -151,16 +250,47 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
   }
   __syncthreads();
 
-  {
-    uint32_t c[W];
-    uint32_t run = 0;
+  // Thread t < 256 handles digit t: the waves' offsets inside the tile and
+  // the digit's start in the tile.
+  uint32_t run = 0, total = 0;
+  if (t < kSortBins) {
 #pragma unroll
-    for (int q = 0; q < W; ++q) { c[q] = wc[q][t]; wc[q][t] = run; run += c[q]; }
-    uint32_t tot;
-    uint32_t ds = block_excl_scan_u32<kSortBlock>(run, scr, &tot);
+    for (int q = 0; q < W; ++q) { const uint32_t c = wc[q][t]; wc[q][t] = run; run += c; }
+#pragma unroll
+    for (int q = 0; q < kSortHistSlots; ++q) total += hist[q * kHistStride + t];
+  }
+  uint32_t tot;
+  const uint32_t ds = block_excl_scan_u32<kSortBlock>(run, scr, &tot);
+  const uint32_t dbase = block_excl_scan_u32<kSortBlock>(total, scr, &tot);
+  if (t < kSortBins) {
     dstart[t] = ds;
-    const uint32_t dbase = block_excl_scan_u32<kSortBlock>(coltot[t], scr, &tot);   // keys of smaller digits
-    gbase[t] = dbase + off[(int64_t)t * ntiles + blockIdx.x];
+    if (!last) in_group();
+    // keys of digit t in the earlier groups: their words backwards until a
+    // running total, kLbWindow per round of loads
+    uint32_t gp = 0;
+    for (int64_t q = g - 1; q >= 0 && !out;) {
+      uint32_t wv[kLbWindow];
+#pragma unroll
+      for (int j = 0; j < kLbWindow; ++j)
+        wv[j] = q - j >= 0 ? __hip_atomic_load(&gw[(q - j) * kSortBins + t], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT)
+                           : kLbPrefix;   // before group 0: nothing
+      int j = 0;
+      bool done = false;
+      for (; j < kLbWindow; ++j) {
+        const uint32_t kind = wv[j] & kLbKind;
+        if (kind == 0u) break;             // not published yet: reload from here
+        gp += wv[j] & kLbCount;
+        if (kind == kLbPrefix) { done = true; break; }
+      }
+      if (done) break;
+      q -= j;
+      if (j < kLbWindow) wait();
+    }
+    if (last && g > 0)
+      __hip_atomic_store(&gw[g * kSortBins + t], kLbPrefix | (gp + ip + run), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    gbase[t] = dbase + gp + ip;
   }
   __syncthreads();
 #pragma unroll
@@ -190,22 +320,26 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(const K* __restric
 }
 
 template <typename K>
-bool radix_sort_impl(K* k0, uint32_t* v0, K* k1, uint32_t* v1, int64_t n, int bits, uint32_t* scratch,
-                     hipStream_t s) {
-  if (n <= 0) return false;
-  const int64_t ntiles = div_up(n, kSortTile);
-  uint32_t* hist = scratch;
-  uint32_t* coltot = scratch + kSortBins * ntiles;
+bool radix_sort_impl(K* k0, uint32_t* v0, K* k1, uint32_t* v1, int64_t n, int lo, int bits, uint32_t* scratch,
+                     int32_t* err, hipStream_t s) {
   const int passes = (int)div_up(bits, 8);
+  if (n <= 0 || passes <= 0) return false;
+  const int64_t ntiles = div_up(n, kSortTile);
+  const size_t pw = sort_pass_words(n);
+  uint32_t* hist = scratch;                                                 // [slots][passes][256]
+  uint32_t* tickets = scratch + kSortHistSlots * kHistStride;               // [passes]
+  uint32_t* words = scratch + kSortHeader;                                  // [passes][pw]
+  const int64_t nw = kSortHeader + (int64_t)passes * (int64_t)pw;
+  hipLaunchKernelGGL(k_sort_clear, dim3((unsigned)std::min<int64_t>(div_up(nw, kScanBlock), 1024)), dim3(kScanBlock),
+                     0, s, scratch, nw);
+  hipLaunchKernelGGL(k_sort_hist<K>, dim3((unsigned)std::min<int64_t>(div_up(n, (int64_t)kSortBlock * kHistRounds), 256)),
+                     dim3(kSortBlock), 0, s, k0, n, lo, passes, hist);
   K* ki = k0; K* ko = k1;
   uint32_t* vi = v0; uint32_t* vo = v1;
   for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist<K>, dim3((unsigned)ntiles), dim3(kSortBlock), 0, s, ki, n, shift, hist,
-                       ntiles);
-    hipLaunchKernelGGL(k_radix_colscan, dim3(kSortBins), dim3(kSortBlock), 0, s, hist, ntiles, coltot);
-    hipLaunchKernelGGL(k_radix_scatter<K>, dim3((unsigned)ntiles), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n,
-                       shift, hist, ntiles, coltot);
+    uint32_t* tw = words + (size_t)p * pw;
+    hipLaunchKernelGGL(k_sort_pass<K>, dim3((unsigned)ntiles), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n, ntiles,
+                       lo + 8 * p, hist + p * kSortBins, tickets + p, tw, tw + ntiles * kSortBins, err);
     K* tk = ki; ki = ko; ko = tk;
     uint32_t* tv = vi; vi = vo; vo = tv;
   }
@@ -224,14 +358,14 @@ void scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uin
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanBlock), 0, s, in, out, n, scratch, nb, total);
 }
 
-bool radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int bits,
-                    uint32_t* scratch, hipStream_t s) {
-  return radix_sort_impl<uint64_t>(k0, v0, k1, v1, n, bits, scratch, s);
+bool radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int lo, int bits,
+                    uint32_t* scratch, int32_t* err, hipStream_t s) {
+  return radix_sort_impl<uint64_t>(k0, v0, k1, v1, n, lo, bits, scratch, err, s);
 }
 
-bool radix_sort_u32(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int bits,
-                    uint32_t* scratch, hipStream_t s) {
-  return radix_sort_impl<uint32_t>(k0, v0, k1, v1, n, bits, scratch, s);
+bool radix_sort_u32(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int lo, int bits,
+                    uint32_t* scratch, int32_t* err, hipStream_t s) {
+  return radix_sort_impl<uint32_t>(k0, v0, k1, v1, n, lo, bits, scratch, err, s);
 }
 
 }  // namespace nbx

@@ -178,40 +178,45 @@ __global__ __launch_bounds__(kBlock) void k_keys2(int64_t n, StateArrays st, Sys
   key2[i] = lo;
 }
 
-// After the sort by the first key word, bodies of one system whose first
-// words are equal (they share all 21 digits) sit next to each other, in
-// their previous order.  Each such run is sorted in place by the second key
-// word (digits 21..41), stably: the order a sort by (system, first word,
-// second word) with ties in the previous order gives.  The second word is
-// written for the runs' bodies only, the only ones whose second word is
-// ever read (where two neighbours' first words are equal).
-__global__ __launch_bounds__(kBlock) void k_fix_runs(int64_t n, StateArrays st, SysArrays sy,
-                                                     const uint64_t* __restrict__ key, uint64_t* __restrict__ key2,
-                                                     uint32_t* __restrict__ amag) {
+// The sort orders bodies by bits [lo, 63) of the first key word (its top
+// digits: enough to separate the bodies of a tree of the last build's depth).
+// Bodies of one system whose words agree on those bits sit next to each
+// other, in their previous order.  Each such run is sorted in place by the
+// whole first word, then the second word (digits 21..41), stably: the order a
+// sort by (system, first word, second word) with ties in the previous order
+// gives.  The second word is written for the runs' bodies only, the only ones
+// whose second word is ever read (where two neighbours' first words are
+// equal).  Runs are disjoint and a swap inside one keeps its bits [lo, 63), so
+// the run bounds every thread finds do not depend on the others' swaps.
+__global__ __launch_bounds__(kBlock) void k_fix_runs(int64_t n, StateArrays st, SysArrays sy, uint64_t* key,
+                                                     uint64_t* __restrict__ key2, uint32_t* __restrict__ amag,
+                                                     int lo) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int32_t s = st.sys[i];
-  const uint64_t k = key[i];
-  const bool eq_next = i + 1 < n && st.sys[i + 1] == s && key[i + 1] == k;
-  const bool eq_prev = i > 0 && st.sys[i - 1] == s && key[i - 1] == k;
+  const uint64_t top = key[i] >> lo;
+  auto same = [&](int64_t j) { return st.sys[j] == s && (key[j] >> lo) == top; };
+  const bool eq_next = i + 1 < n && same(i + 1);
+  const bool eq_prev = i > 0 && same(i - 1);
   if (!eq_next || eq_prev) return;                           // not the first body of a run
   int64_t j = i + 1;                                         // the run is [i, j]
-  while (j + 1 < n && st.sys[j + 1] == s && key[j + 1] == k) ++j;
+  while (j + 1 < n && same(j + 1)) ++j;
   double root[4];
   sys_root(sy, s, root);
   for (int64_t t = i; t <= j; ++t) {
-    uint64_t hi, lo;
-    key_digits(st.x[t], st.y[t], st.z[t], root, kKeyDigits, kMaxLevels, hi, lo);
-    key2[t] = lo;
+    uint64_t hi, lo2;
+    key_digits(st.x[t], st.y[t], st.z[t], root, kKeyDigits, kMaxLevels, hi, lo2);
+    key2[t] = lo2;
   }
   for (int64_t t = i + 1; t <= j; ++t)
-    for (int64_t u = t; u > i && key2[u - 1] > key2[u]; --u) {
+    for (int64_t u = t; u > i && (key[u - 1] > key[u] || (key[u - 1] == key[u] && key2[u - 1] > key2[u])); --u) {
       const int64_t v = u - 1;
       auto sw_d = [&](double* a) { const double w = a[u]; a[u] = a[v]; a[v] = w; };
       auto sw_i = [&](int32_t* a) { const int32_t w = a[u]; a[u] = a[v]; a[v] = w; };
+      auto sw_k = [&](uint64_t* a) { const uint64_t w = a[u]; a[u] = a[v]; a[v] = w; };
       sw_d(st.x); sw_d(st.y); sw_d(st.z); sw_d(st.vx); sw_d(st.vy); sw_d(st.vz); sw_d(st.m);
       sw_i(st.type); sw_i(st.id);
-      const uint64_t w = key2[u]; key2[u] = key2[v]; key2[v] = w;
+      sw_k(key); sw_k(key2);
       if (amag) { const uint32_t a = amag[u]; amag[u] = amag[v]; amag[v] = a; }
     }
 }
@@ -325,35 +330,41 @@ __global__ __launch_bounds__(kBlock) void k_lcp_count(int64_t n, const uint64_t*
     if (eq_i) atomicAdd(&hist[kPartEqual], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < kPartW) part[(int64_t)blockIdx.x * kPartW + threadIdx.x] = hist[threadIdx.x];
+  if (threadIdx.x < kPartW) part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Column `col` of the partial rows, one block per column: a per-depth column
-// becomes its exclusive prefix over the blocks (in place) and its total is
-// the node count of that depth; the last two columns reduce to the deepest
-// leaf and the equal-key pairs.
+// Column `col` of the partial counts, one block per column: a per-depth
+// column becomes its exclusive prefix over the blocks (in place) and its
+// total is the node count of that depth; the last two columns reduce to the
+// deepest leaf and the equal-key pairs.
 __global__ __launch_bounds__(kBlock) void k_lcp_scan(uint32_t* __restrict__ part, int64_t rows, int has_key2,
                                                      DevScalars* sc) {
   __shared__ uint32_t scr[kBlock / kWave + 1];
   const int col = blockIdx.x;
+  uint32_t* cp = part + (int64_t)col * rows;
   if (col < kPartMaxDepth) {
-    uint32_t carry = 0;
-    for (int64_t c = 0; c < rows; c += kBlock) {
-      const int64_t r = c + threadIdx.x;
-      const uint32_t v = r < rows ? part[r * kPartW + col] : 0u;
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan_u32<kBlock>(v, scr, &tot);
-      if (r < rows) part[r * kPartW + col] = carry + ex;
-      carry += tot;
+    // thread t takes rows [t*per, (t+1)*per) of the column: their sum, one
+    // block scan of the sums, then the rows' running prefix
+    const int64_t per = (rows + kBlock - 1) / kBlock;
+    const int64_t r0 = (int64_t)threadIdx.x * per;
+    const int64_t r1 = r0 + per < rows ? r0 + per : rows;
+    uint32_t sum = 0;
+    for (int64_t r = r0; r < r1; ++r) sum += cp[r];
+    uint32_t tot;
+    uint32_t run = block_excl_scan_u32<kBlock>(sum, scr, &tot);
+    for (int64_t r = r0; r < r1; ++r) {
+      const uint32_t v = cp[r];
+      cp[r] = run;
+      run += v;
     }
-    if (threadIdx.x == 0 && col < kPartEnds) sc->level_count[col] = (int32_t)carry;
+    if (threadIdx.x == 0 && col < kPartEnds) sc->level_count[col] = (int32_t)tot;
     return;
   }
   __shared__ uint32_t red[kBlock / kWave];
   const bool is_max = col == kPartMaxDepth;
   uint32_t v = 0;
   for (int64_t r = threadIdx.x; r < rows; r += kBlock) {
-    const uint32_t x = part[r * kPartW + col];
+    const uint32_t x = cp[r];
     v = is_max ? (x > v ? x : v) : v + x;
   }
 #pragma unroll
@@ -398,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
     pre = pre_off[i];
     skipv = (i + 1 < n) ? pre_off[i + 1] : (uint32_t)sc->total_nodes;
   }
-  place_internal_nodes(valid, lp, li, pre, skipv, part + (int64_t)blockIdx.x * kPartW, sc, level_list, end_list,
+  place_internal_nodes(valid, lp, li, pre, skipv, part, sc, level_list, end_list,
                        cap_int);
   if (!valid) return;
   const double s0 = sy.root[4 * st.sys[i] + 3];
@@ -469,6 +480,7 @@ __global__ __launch_bounds__(kWave) void k_check(DevScalars* sc, int64_t cap_rec
     if (sc->max_leaf_depth - 1 > dmax) f |= kAbortDepth;
     if (sc->collisions > 0 || sc->coincident > 0) f |= kAbortCollision;
     if (sc->err & 1) f |= kAbortNonFinite;
+    if (sc->err & 2) f |= kAbortSort;
     if (f) sc->abort_flags |= f;
   }
 }
@@ -999,9 +1011,9 @@ void launch_keys_export(int64_t n, const StateArrays& st, const SysArrays& sy, u
                         hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_keys_export, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, hi, lo);
 }
-void launch_fix_runs(int64_t n, const StateArrays& st, const SysArrays& sy, const uint64_t* key, uint64_t* key2,
-                     uint32_t* amag, hipStream_t s) {
-  if (n > 1) hipLaunchKernelGGL(k_fix_runs, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, key, key2, amag);
+void launch_fix_runs(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint64_t* key2,
+                     uint32_t* amag, int lo, hipStream_t s) {
+  if (n > 1) hipLaunchKernelGGL(k_fix_runs, dim3(grid_of(n)), dim3(kBlock), 0, s, n, st, sy, key, key2, amag, lo);
 }
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(kBlock), 0, s, n, idx);

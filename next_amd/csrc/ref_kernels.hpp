@@ -39,8 +39,8 @@ void launch_keys2(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_
 // Runs of bodies of one system with equal first key words (in the order of
 // the sort by the first word) sorted in place by the second word, stably;
 // key2 written for those bodies (amag, may be null, moves with them).
-void launch_fix_runs(int64_t n, const StateArrays& st, const SysArrays& sy, const uint64_t* key, uint64_t* key2,
-                     uint32_t* amag, hipStream_t s);
+void launch_fix_runs(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint64_t* key2,
+                     uint32_t* amag, int lo, hipStream_t s);
 void launch_iota(int64_t n, uint32_t* idx, hipStream_t s);
 void launch_gather_u64(int64_t n, const uint32_t* perm, const uint64_t* src, uint64_t* dst, hipStream_t s);
 void launch_gather_sys(int64_t n, const uint32_t* perm, const int32_t* sys, uint32_t* dst, hipStream_t s);
