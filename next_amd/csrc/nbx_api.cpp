@@ -324,8 +324,7 @@ const uint64_t* sort_bodies(nbx_ctx* c, int digits) {
   hipStream_t s = c->stream;
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
-  launch_bbox(St, P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), P<double>(c->box_part), sy, s);
-  launch_fill_scalars(c->sc, 1 | 4, s);
+  launch_bbox(St, P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), P<double>(c->box_part), sy, c->sc, s);
   launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
   record(c, EV_KEYS);
   uint32_t* scr = P<uint32_t>(c->scratch);
@@ -405,36 +404,30 @@ nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
 
 // Reference node records of the current tree (layout and skip pointers;
 // the moments come from the upward pass): k_emit (unless `emitted`: already
-// run into buffers that hold the tree), the two depth sorts and k_pair.
-// Needs the scalars of the last build on the host.
+// run into buffers that hold the tree).  Needs the scalars of the last
+// build on the host.
 nbx_status ref_layout(nbx_ctx* c, bool emitted) {
   const int64_t n = c->n, total = c->total_nodes;
-  const int64_t m = internal_count(*c->hsc);
   hipStream_t s = c->stream;
   // The walk reads node records at 32-bit byte offsets, and one record past
   // the last node (a successor it loads but never uses).
   NBXCHK(ensure(c, c->nodes, (size_t)(total + 1) * sizeof(RefNode)));
   NBXCHK(ensure(c, c->heads, (size_t)(total + 1) * sizeof(WalkHead)));
-  uint32_t* scr = P<uint32_t>(c->scratch);
   RefNode* nodes = P<RefNode>(c->nodes);
   if (!emitted)
     launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), view(c, c->cur), sysa(c),
                 c->sc, nodes, P<WalkHead>(c->heads), P<int32_t>(c->level_list), P<uint32_t>(c->end_list),
                 std::min(cap_words(c->nodes, sizeof(RefNode)), cap_words(c->heads, sizeof(WalkHead))), cap_int(c), s);
-  launch_pair(m, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes, s);
   return NBX_OK;
 }
 
-// Second half: wait for the scalars, then (rarely) the two-word key path,
-// node layout and multipoles.
+// Second half: wait for the scalars, then the node layout and multipoles.
 nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
   const int64_t n = c->n;
-  const int S = (int)c->sys_n.size();
   hipStream_t s = c->stream;
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
-  uint32_t* tot_nodes = reinterpret_cast<uint32_t*>(&c->sc->total_nodes);
   HIPCHK(c, hipEventSynchronize(c->ev_scalars));
   ++c->tim.host_waits;
   if (c->hsc->err & 1) return fail(c, NBX_ERR_NONFINITE, "a particle position is NaN or infinite");
@@ -504,10 +497,8 @@ nbx_status build_post(nbx_ctx* c, const uint64_t* skey) {
     NBXCHK(ref_layout(c, c->spec_emit && total + 1 <= (int64_t)c->spec_nodes && m <= (int64_t)c->spec_int));
     record(c, EV_BUILD);
     RefNode* nodes = P<RefNode>(c->nodes);
-    for (int d = kMaxDepthSlots - 1; d >= 0; --d)
-      if (c->hsc->level_count[d] > 0)
-        launch_upward(P<int32_t>(c->level_list), off[d], c->hsc->level_count[d], nodes, P<WalkHead>(c->heads),
-                      c->cfg.theta, s);
+    launch_upward_all(c->hsc->level_count, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes,
+                      P<WalkHead>(c->heads), c->cfg.theta, St.sys, sy.root, s);
   }
   record(c, EV_UPWARD);
   c->tree_valid = true;
@@ -560,7 +551,6 @@ int64_t group_grid(const nbx_ctx* c) { return c->ntgroups >= 0 ? c->ntgroups : c
 nbx_status walk(nbx_ctx* c, bool kick) {
   const bool count = (c->prof & 2) != 0;
   if (count) launch_fill_scalars(c->sc, 8, c->stream);
-  if (kick) launch_fill_v2max(sysa(c), c->stream);
   record(c, EV_WALK0);
   if (c->physics)
     launch_phys_walk(P<Group>(c->tgroups), group_grid(c), P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
@@ -740,12 +730,11 @@ void build_bounded(nbx_ctx* c) {
     RefNode* nodes = P<RefNode>(c->nodes);
     launch_emit(n, P<int32_t>(c->lcp), P<uint32_t>(c->cnt_s), P<uint32_t>(c->lvl_part), St, sy, c->sc, nodes,
                 P<WalkHead>(c->heads), P<int32_t>(c->level_list), P<uint32_t>(c->end_list), cap_rec, cap_lst, s);
-    launch_pair_dev(c->b_grid_pair, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes, c->sc, s);
-    launch_upward_tail(c->b_dmax + 1, P<int32_t>(c->level_list), nodes, P<WalkHead>(c->heads), c->cfg.theta, c->sc,
-                       s);
+    launch_upward_tail(c->b_dmax + 1, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes,
+                       P<WalkHead>(c->heads), c->cfg.theta, St.sys, sy.root, c->sc, s);
     for (int d = c->b_dmax; d >= 0; --d)
-      launch_upward_level(c->b_grid_up[d], d, P<int32_t>(c->level_list), nodes, P<WalkHead>(c->heads), c->cfg.theta,
-                          c->sc, s);
+      launch_upward_level(c->b_grid_up[d], d, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), nodes,
+                          P<WalkHead>(c->heads), c->cfg.theta, St.sys, sy.root, c->sc, s);
   }
   c->skey = skey;
   c->skey2 = P<uint64_t>(c->key2[0]);
@@ -1518,7 +1507,7 @@ nbx_status nbx_debug_keys(nbx_ctx* c, double* root4, uint64_t* hi, uint64_t* lo)
   if (!(hi && lo)) return fail(c, NBX_ERR_ARG, "null key output");
   SysArrays sy = sysa(c);
   launch_bbox(view(c, c->cur), P<BoxBlock>(c->boxes), c->nboxes, P<int32_t>(c->box_off), P<double>(c->box_part), sy,
-              c->stream);
+              nullptr, c->stream);
   for (int w = 0; w < 2; ++w) NBXCHK(ensure(c, c->key2[w], (size_t)c->cap * 8));
   launch_keys_export(n, view(c, c->cur), sy, P<uint64_t>(c->key2[0]), P<uint64_t>(c->key2[1]), c->stream);
   HIPCHK(c, hipMemcpyAsync(root4, c->s_root.p, 32, hipMemcpyDeviceToHost, c->stream));
@@ -1591,18 +1580,8 @@ nbx_status nbx_debug_tree(nbx_ctx* c, int32_t* depth, int32_t* leaf, int32_t* bo
     // The physics build writes walk records only: lay out the reference
     // records of the same tree, with NEXT's moments.
     NBXCHK(ref_layout(c, false));
-    int64_t off = 0;
-    for (int d = 0; d < kMaxDepthSlots; ++d) {
-      const int64_t cnt = c->hsc->level_count[d];
-      off += cnt;
-    }
-    for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
-      const int64_t cnt = c->hsc->level_count[d];
-      off -= cnt;
-      if (cnt > 0)
-        launch_upward(P<int32_t>(c->level_list), off, cnt, P<RefNode>(c->nodes), P<WalkHead>(c->heads), c->cfg.theta,
-                      c->stream);
-    }
+    launch_upward_all(c->hsc->level_count, P<int32_t>(c->level_list), P<uint32_t>(c->end_list), P<RefNode>(c->nodes),
+                      P<WalkHead>(c->heads), c->cfg.theta, view(c, c->cur).sys, sysa(c).root, c->stream);
   }
   // one staging allocation for all outputs
   const size_t ni = (size_t)total * 4, nd = (size_t)total * 8;

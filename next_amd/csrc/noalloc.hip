@@ -187,7 +187,7 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
                      n, x, y, z, m, type, St, A.at<Group>(kLayout.groups), A.at<BoxBlock>(kLayout.boxes),
                      A.at<int32_t>(kLayout.box_off), A.at<int32_t>(kLayout.soff));
   launch_bbox(St, A.at<BoxBlock>(kLayout.boxes), (int)nb, A.at<int32_t>(kLayout.box_off),
-              A.at<double>(kLayout.box_part), sy, s);
+              A.at<double>(kLayout.box_part), sy, nullptr, s);
   launch_fill_scalars(sc, 1 | 4 | 8, s);
   launch_set_run(sc, 0, s);                                // no abort flag: the walk runs
   uint64_t* key[2] = {A.at<uint64_t>(kLayout.key[0]), A.at<uint64_t>(kLayout.key[1])};
@@ -225,13 +225,7 @@ nbx_status accel_in_arena(const Arena& A, int64_t n, const double* x, const doub
   uint32_t* end_list = A.at<uint32_t>(kLayout.end_list);
   launch_emit(n, lcp, cnt_s, part, St, sy, sc, nodes, heads, level_list, end_list, (uint32_t)(kNaNodes + 1),
               (uint32_t)kNaNodes, s);
-  launch_pair(mi, level_list, end_list, nodes, s);
-  int64_t off = mi;
-  for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
-    const int64_t cnt = h.level_count[d];
-    off -= cnt;
-    if (cnt > 0) launch_upward(level_list, off, cnt, nodes, heads, theta, s);
-  }
+  launch_upward_all(h.level_count, level_list, end_list, nodes, heads, theta, St.sys, sy.root, s);
   double* acc[3] = {A.at<double>(kLayout.acc[0]), A.at<double>(kLayout.acc[1]), A.at<double>(kLayout.acc[2])};
   launch_walk(A.at<Group>(kLayout.groups), div_up(n, kWave), nodes, heads, St, sy, theta, acc[0], acc[1], acc[2], sc,
               false, false, s);

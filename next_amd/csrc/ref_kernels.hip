@@ -100,10 +100,17 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(StateArrays st, const B
 
 // Root centre and half-width of each system from its blocks' partials
 // (step.h:93-100), one wave per system; min and max are exact, so the
-// order does not matter.
+// order does not matter.  clear (may be null): scalars to reset for the
+// tree build that follows.
 __global__ __launch_bounds__(kWave) void k_bbox_final(const double* __restrict__ part,
-                                                      const int32_t* __restrict__ block_off, SysArrays sy) {
+                                                      const int32_t* __restrict__ block_off, SysArrays sy,
+                                                      DevScalars* clear) {
   const int s = blockIdx.x;
+  if (clear && s == 0) {     // a tree build's counts and flags (k_fill_scalars, what = 1 | 4)
+    const int t = threadIdx.x;
+    clear->level_count[t] = 0;
+    if (t == 0) { clear->collisions = 0; clear->coincident = 0; clear->max_leaf_depth = 0; clear->err = 0; }
+  }
   if (s >= sy.nsys) return;
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};
   for (int b = block_off[s] + (int)threadIdx.x; b < block_off[s + 1]; b += kWave)
@@ -412,57 +419,30 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
   place_internal_nodes(valid, lp, li, pre, skipv, part, sc, level_list, end_list,
                        cap_int);
   if (!valid) return;
-  const double s0 = sy.root[4 * st.sys[i] + 3];
+  // Internal nodes are written by the upward pass (their records, skip
+  // pointers included, come from the node lists placed above).
   const int ns = li > lp ? li - lp : 0;
-  for (int t = 0; t < ns; ++t) {
-    const int d = lp + 1 + t;
-    const uint32_t p = pre + t;
-    if (p >= cap_nodes) break;
-    // mass, centre of mass, softening numerator and quadrupole come from
-    // k_upward, the skip pointer from k_pair
-    RefNode& nd = nodes[p];
-    nd.size = ldexp(s0, -d);
-    nd.body = -1;
-    nd.depth = d;
-    nd.first = (int32_t)i;
-  }
   if (pre + ns < cap_nodes) {
     const uint32_t p = pre + ns;
     const int d = 1 + (li > lp ? li : lp);
-    // Leaves carry no quadrupole; the walk never reads a leaf's Q fields.
-    RefNode& nd = nodes[p];
     const double m = st.m[i];
-    const double size = ldexp(s0, -d);
-    nd.cx = st.x[i]; nd.cy = st.y[i]; nd.cz = st.z[i];
+    const double size = ldexp(sy.root[4 * st.sys[i] + 3], -d);
+    // Leaves carry no quadrupole; the walk never reads a leaf's Q fields
+    // (the first 80 bytes of the record are written, in one piece).
+    const double px = st.x[i], py = st.y[i], pz = st.z[i];
+    RefNode& nd = nodes[p];
+    nd.cx = px; nd.cy = py; nd.cz = pz;
     nd.m = m;
-    nd.size = size;
-    // softening.h:26-27: size*0.015 + cbrt(m)*0.002, summed before the taper
-    nd.esum = size * 0.015 + cbrt(m) * 0.002;
     nd.skip = (int32_t)(p + 1);
     nd.body = (int32_t)i;
     nd.depth = d;
     nd.first = (int32_t)i;
+    nd.size = size;
+    // softening.h:26-27: size*0.015 + cbrt(m)*0.002, summed before the taper
+    nd.esum = size * 0.015 + cbrt(m) * 0.002;
     // a leaf is accepted whatever its distance: no threshold
-    heads[p] = WalkHead{nd.cx, nd.cy, nd.cz, 0.0f, m == 0.0 ? (int32_t)(p + 1) | kLinkZeroMass : ~(int32_t)i};
+    heads[p] = WalkHead{px, py, pz, 0.0f, m == 0.0 ? (int32_t)(p + 1) | kLinkZeroMass : ~(int32_t)i};
   }
-}
-
-// Skip pointer of the k-th internal node of the per-depth list.
-__global__ __launch_bounds__(kBlock) void k_pair(int64_t m, const int32_t* __restrict__ level_list,
-                                                 const uint32_t* __restrict__ end_list, RefNode* __restrict__ nodes) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= m) return;
-  nodes[level_list[k]].skip = (int32_t)end_list[k];
-}
-
-// The same with the count on the device (fixed grid striding over it).
-__global__ __launch_bounds__(kBlock) void k_pair_dev(const int32_t* __restrict__ level_list,
-                                                     const uint32_t* __restrict__ end_list,
-                                                     RefNode* __restrict__ nodes, const DevScalars* __restrict__ sc) {
-  if (sc->abort_flags) return;
-  const int64_t m = (int64_t)sc->level_off[kMaxDepthSlots - 1] + sc->level_count[kMaxDepthSlots - 1];
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < m; k += (int64_t)gridDim.x * kBlock)
-    nodes[level_list[k]].skip = (int32_t)end_list[k];
 }
 
 // A build whose counts stay on the device: each depth's offset in the node
@@ -485,24 +465,27 @@ __global__ __launch_bounds__(kWave) void k_check(DevScalars* sc, int64_t cap_rec
   }
 }
 
-// Mass, centre of mass and one-level softened quadrupole of the internal
-// nodes of one depth from their children (octree.h:104-126), and the
-// squared-distance thresholds of the walk's opening test for this theta.
-__device__ __forceinline__ void upward_node(int32_t id, RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
-                                            double inv_theta) {
-  const int32_t end = nodes[id].skip;
-  const double size = nodes[id].size;
+// Mass, centre of mass and one-level softened quadrupole of an internal node
+// of depth d from its children (octree.h:104-126), and the squared-distance
+// thresholds of the walk's opening test for this theta: the node's record
+// (all of it but the padding) and its walk head.  end: the preorder index after its
+// subtree (its skip pointer, from the list of skip targets); the first body
+// comes from the first child, the half-width from the system's root.
+__device__ __forceinline__ void upward_node(int32_t id, int32_t end, int d, RefNode* __restrict__ nodes,
+                                            WalkHead* __restrict__ heads, double inv_theta,
+                                            const int32_t* __restrict__ sys, const double* __restrict__ root) {
   // One walk of the child chain (at most 8; the bound also keeps a malformed
   // chain finite) gathers each child's mass and centre into registers; a
   // slot past the last child keeps mass 0, which both sums skip.
   double cmv[8], cxv[8], cyv[8], czv[8];
-  int32_t c = id + 1;
+  int32_t c = id + 1, first = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     cmv[j] = 0.0; cxv[j] = 0.0; cyv[j] = 0.0; czv[j] = 0.0;
     if (c < end) {
       const RefNode& ch = nodes[c];
       cmv[j] = ch.m; cxv[j] = ch.cx; cyv[j] = ch.cy; czv[j] = ch.cz;
+      if (j == 0) first = ch.first;
       c = ch.skip;
     }
   }
@@ -518,6 +501,7 @@ __device__ __forceinline__ void upward_node(int32_t id, RefNode* __restrict__ no
   }
   double cx = sx, cy = sy, cz = sz;
   if (M > 0.0) { cx = sx / M; cy = sy / M; cz = sz / M; }
+  const double size = ldexp(root[4 * sys[first] + 3], -d);
   double qxx = 0, qyy = 0, qzz = 0, qxy = 0, qxz = 0, qyz = 0;
   const double soft = size * size * 0.01;
 #pragma unroll
@@ -533,22 +517,35 @@ __device__ __forceinline__ void upward_node(int32_t id, RefNode* __restrict__ no
     qxz = qxz + mc * (3.0 * rx * rz);
     qyz = qyz + mc * (3.0 * ry * rz);
   }
-  RefNode& p = nodes[id];
-  p.m = M; p.cx = cx; p.cy = cy; p.cz = cz;
+  RefNode p;
+  p.cx = cx; p.cy = cy; p.cz = cz; p.m = M;
+  p.skip = end;
+  p.body = -1;
+  p.depth = d;
+  p.first = first;
+  p.size = size;
   p.esum = size * 0.015 + cbrt(M) * 0.002;
   p.qxx = qxx; p.qyy = qyy; p.qzz = qzz; p.qxy = qxy; p.qxz = qxz; p.qyz = qyz;
+  RefNode& out = nodes[id];                  // the record but its padding
+  out.cx = p.cx; out.cy = p.cy; out.cz = p.cz; out.m = p.m;
+  out.skip = p.skip; out.body = p.body; out.depth = p.depth; out.first = p.first;
+  out.size = p.size; out.esum = p.esum;
+  out.qxx = p.qxx; out.qyy = p.qyy; out.qzz = p.qzz; out.qxy = p.qxy; out.qxz = p.qxz; out.qyz = p.qyz;
   const double q = size * inv_theta;
   const double q2 = q * q;
   const float acc2 = (q2 > 1e-36 && q2 < 1e36) ? __double2float_ru(q2 * (1.0 + 2e-6)) : __builtin_nanf("");
   heads[id] = WalkHead{cx, cy, cz, acc2, M == 0.0 ? (end | kLinkZeroMass) : end};
 }
 
-__global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list, int64_t beg, int64_t cnt,
-                                                   RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
-                                                   double inv_theta) {
+// The internal nodes [beg, beg + cnt) of the node list, all of depth d.
+__global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ level_list,
+                                                   const uint32_t* __restrict__ end_list, int64_t beg, int64_t cnt,
+                                                   int d, RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
+                                                   double inv_theta, const int32_t* __restrict__ sys,
+                                                   const double* __restrict__ root) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= cnt) return;
-  upward_node(level_list[beg + k], nodes, heads, inv_theta);
+  upward_node(level_list[beg + k], (int32_t)end_list[beg + k], d, nodes, heads, inv_theta, sys, root);
 }
 
 // Depths d_lo .. 41 of a build whose counts stay on the device, deepest
@@ -557,13 +554,17 @@ __global__ __launch_bounds__(kBlock) void k_upward(const int32_t* __restrict__ l
 // of a tree holds few nodes, chains mostly, and its depth varies from build
 // to build.
 __global__ __launch_bounds__(kBlock) void k_upward_tail(int d_lo, const int32_t* __restrict__ level_list,
+                                                        const uint32_t* __restrict__ end_list,
                                                         RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
-                                                        double inv_theta, const DevScalars* __restrict__ sc) {
+                                                        double inv_theta, const int32_t* __restrict__ sys,
+                                                        const double* __restrict__ root,
+                                                        const DevScalars* __restrict__ sc) {
   if (sc->abort_flags) return;
   for (int d = kMaxLevels - 1; d >= d_lo; --d) {
     const int64_t beg = sc->level_off[d], cnt = sc->level_count[d];
     if (cnt == 0) continue;
-    for (int64_t k = threadIdx.x; k < cnt; k += kBlock) upward_node(level_list[beg + k], nodes, heads, inv_theta);
+    for (int64_t k = threadIdx.x; k < cnt; k += kBlock)
+      upward_node(level_list[beg + k], (int32_t)end_list[beg + k], d, nodes, heads, inv_theta, sys, root);
     __syncthreads();
   }
 }
@@ -572,12 +573,15 @@ __global__ __launch_bounds__(kBlock) void k_upward_tail(int d_lo, const int32_t*
 // depth's extent in the node list comes from the scalars (k_check), the grid
 // is fixed and strides over it.
 __global__ __launch_bounds__(kBlock) void k_upward_level(int d, const int32_t* __restrict__ level_list,
+                                                         const uint32_t* __restrict__ end_list,
                                                          RefNode* __restrict__ nodes, WalkHead* __restrict__ heads,
-                                                         double inv_theta, const DevScalars* __restrict__ sc) {
+                                                         double inv_theta, const int32_t* __restrict__ sys,
+                                                         const double* __restrict__ root,
+                                                         const DevScalars* __restrict__ sc) {
   if (sc->abort_flags) return;
   const int64_t beg = sc->level_off[d], cnt = sc->level_count[d];
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < cnt; k += (int64_t)gridDim.x * kBlock)
-    upward_node(level_list[beg + k], nodes, heads, inv_theta);
+    upward_node(level_list[beg + k], (int32_t)end_list[beg + k], d, nodes, heads, inv_theta, sys, root);
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so that each XCD walks
@@ -924,6 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_dt(SysArrays sy, double base, double
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= sy.nsys) return;
   const double v2 = __longlong_as_double((long long)sy.v2max[s]);
+  sy.v2max[s] = 0ull;   // read once: the next kicks gather the next step's maximum
   double v = sqrt(v2);
   if (1e4 < v) v = 1e4;
   double dt;
@@ -996,9 +1001,10 @@ void launch_set_dt(const SysArrays& sy, double dt, hipStream_t s) {
   hipLaunchKernelGGL(k_set_dt, dim3(sys_grid(sy)), dim3(kBlock), 0, s, sy, dt);
 }
 void launch_bbox(const StateArrays& st, const BoxBlock* blocks, int nblocks, const int32_t* block_off, double* part,
-                 const SysArrays& sy, hipStream_t s) {
+                 const SysArrays& sy, DevScalars* clear, hipStream_t s) {
   if (nblocks > 0) hipLaunchKernelGGL(k_bbox_partial, dim3(nblocks), dim3(kBlock), 0, s, st, blocks, part);
-  hipLaunchKernelGGL(k_bbox_final, dim3((unsigned)std::max(sy.nsys, 1)), dim3(kWave), 0, s, part, block_off, sy);
+  hipLaunchKernelGGL(k_bbox_final, dim3((unsigned)std::max(sy.nsys, 1)), dim3(kWave), 0, s, part, block_off, sy,
+                     clear);
 }
 void launch_keys(int64_t n, const StateArrays& st, const SysArrays& sy, uint64_t* key, uint32_t* idx,
                  DevScalars* scw, hipStream_t s) {
@@ -1059,33 +1065,40 @@ void launch_emit(int64_t n, const int32_t* lcp, const uint32_t* pre_off, const u
     hipLaunchKernelGGL(k_emit, dim3(grid_of(n)), dim3(kBlock), 0, s, n, lcp, pre_off, part, st, sy, sc, nodes, heads,
                        level_list, end_list, cap_nodes, cap_int);
 }
-void launch_pair(int64_t m, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes, hipStream_t s) {
-  if (m > 0) hipLaunchKernelGGL(k_pair, dim3(grid_of(m)), dim3(kBlock), 0, s, m, level_list, end_list, nodes);
-}
-void launch_pair_dev(unsigned grid, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes,
-                     const DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_pair_dev, dim3(grid), dim3(kBlock), 0, s, level_list, end_list, nodes, sc);
-}
 void launch_set_run(DevScalars* sc, int32_t step, hipStream_t s) {
   hipLaunchKernelGGL(k_set_run, dim3(1), dim3(1), 0, s, sc, step);
 }
 void launch_check(DevScalars* sc, int64_t cap_records, int64_t cap_list, int dmax, hipStream_t s) {
   hipLaunchKernelGGL(k_check, dim3(1), dim3(kWave), 0, s, sc, cap_records, cap_list, dmax);
 }
-void launch_upward_tail(int d_lo, const int32_t* level_list, RefNode* nodes, WalkHead* heads, double theta,
-                        const DevScalars* sc, hipStream_t s) {
+void launch_upward_tail(int d_lo, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes,
+                        WalkHead* heads, double theta, const int32_t* sys, const double* root, const DevScalars* sc,
+                        hipStream_t s) {
   if (d_lo < kMaxLevels)
-    hipLaunchKernelGGL(k_upward_tail, dim3(1), dim3(kBlock), 0, s, d_lo, level_list, nodes, heads, 1.0 / theta, sc);
+    hipLaunchKernelGGL(k_upward_tail, dim3(1), dim3(kBlock), 0, s, d_lo, level_list, end_list, nodes, heads,
+                       1.0 / theta, sys, root, sc);
 }
-void launch_upward_level(unsigned grid, int d, const int32_t* level_list, RefNode* nodes, WalkHead* heads,
-                         double theta, const DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_upward_level, dim3(grid), dim3(kBlock), 0, s, d, level_list, nodes, heads, 1.0 / theta, sc);
+void launch_upward_level(unsigned grid, int d, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes,
+                         WalkHead* heads, double theta, const int32_t* sys, const double* root, const DevScalars* sc,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_upward_level, dim3(grid), dim3(kBlock), 0, s, d, level_list, end_list, nodes, heads,
+                     1.0 / theta, sys, root, sc);
 }
-void launch_upward(const int32_t* level_list, int64_t beg, int64_t cnt, RefNode* nodes, WalkHead* heads,
-                   double theta, hipStream_t s) {
+void launch_upward(const int32_t* level_list, const uint32_t* end_list, int64_t beg, int64_t cnt, int d,
+                   RefNode* nodes, WalkHead* heads, double theta, const int32_t* sys, const double* root,
+                   hipStream_t s) {
   if (cnt > 0)
-    hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, beg, cnt, nodes, heads,
-                       1.0 / theta);
+    hipLaunchKernelGGL(k_upward, dim3(grid_of(cnt)), dim3(kBlock), 0, s, level_list, end_list, beg, cnt, d, nodes,
+                       heads, 1.0 / theta, sys, root);
+}
+void launch_upward_all(const int32_t* counts, const int32_t* level_list, const uint32_t* end_list, RefNode* nodes,
+                       WalkHead* heads, double theta, const int32_t* sys, const double* root, hipStream_t s) {
+  int64_t off = 0;
+  for (int d = 0; d < kMaxDepthSlots; ++d) off += counts[d];
+  for (int d = kMaxDepthSlots - 1; d >= 0; --d) {
+    off -= counts[d];
+    launch_upward(level_list, end_list, off, counts[d], d, nodes, heads, theta, sys, root, s);
+  }
 }
 void launch_walk(const Group* groups, int64_t ngroups, const RefNode* nodes, const WalkHead* heads,
                  const StateArrays& st,
