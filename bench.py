@@ -172,12 +172,12 @@ def ensemble(nsys: int, steps: int, warmup: int, ref_body_steps_per_s: float = N
     ratio), N_i log-uniform in [1024, 65536] from seed i, theta = 0.5,
     dt = 1e-3, FP32 walk with FP64 state, all advanced by one set of launches
     per step, the tree rebuilt every REBUILD_EVERY steps and refitted between;
-    leaf and group size chosen by the autotuner first."""
+    leaf size, group size and walk list size chosen by the autotuner first."""
     from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
 
     sizes = ic.ensemble_sizes(nsys)
     systems = [ic.plummer(n, seed=i, virial=None) for i, n in enumerate(sizes)]
-    cands = [(leaf, group) for leaf in (8, 16, 32) for group in (32, 64)]
+    cands = [(leaf, group, lst) for leaf in (8, 16, 32) for group in (32, 64) for lst in (128, 256)]
     with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=16, adaptive_dt=False,
                 rebuild_every=REBUILD_EVERY) as e:
         e.upload_batch(systems)
@@ -204,8 +204,9 @@ def ensemble(nsys: int, steps: int, warmup: int, ref_body_steps_per_s: float = N
                          "host_waits_per_step": tg["host_waits"] / steps, "graph_aborts": tg["graph_aborts"],
                          "steps_from_graph": tg["graph_steps"]},
             "step_by_step_ms_per_step": dt_eager * 1e3 / steps,
-            "autotuned": {"leaf_size": best[0], "group_size": best[1],
-                          "trial_ms_per_step": {f"{lf}/{g}": round(float(m), 3) for (lf, g), m in zip(cands, trial_ms)}},
+            "autotuned": {"leaf_size": best[0], "group_size": best[1], "walk_list": best[2],
+                          "trial_ms_per_step": {f"{lf}/{g}/{ls}": round(float(m), 3)
+                                                for (lf, g, ls), m in zip(cands, trial_ms)}},
             "tree_rebuild_every": REBUILD_EVERY,
             "workload": f"{nsys} Plummer spheres (DF samples), N log-uniform [1024, 65536], theta 0.5 (bmax criterion), "
                         "eps 0.01, dt 1e-3, FP32 walk / FP64 state",
@@ -224,7 +225,8 @@ def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, ener
     from next_amd.nbx import MODE_PHYSICS, PREC_MIXED, Engine
 
     with Engine(mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, theta=theta, leaf_size=best[0],
-                group_size=best[1], adaptive_dt=False, alpha=alpha, rebuild_every=REBUILD_EVERY) as e:
+                group_size=best[1], adaptive_dt=False, alpha=alpha, rebuild_every=REBUILD_EVERY,
+                walk_list=best[2]) as e:
         e.upload_batch(systems)
         e.run(1e-3, 1)                 # the criterion needs the previous accelerations
         a = e.accel()
@@ -278,13 +280,15 @@ def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, ener
                                                if ref_body_steps_per_s else None)}
 
 
-def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, fp64_walk: bool = False) -> dict:
+def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, fp64_walk: bool = False,
+            leaf: int = 16, group: int = 64, walk_list: int = 128) -> dict:
     """Mixed-precision physics mode (FP32 walk, FP64 state) on the headline
     workload: cold uniform sphere of n bodies, eps 0.005, fixed dt 5e-4.  Per
     theta: ms per KDK step, force error against the FP64 direct sum on 20,000
     sampled bodies, and per-body throughput relative to the reference-mode
     step (ref_ms); |dE/E| (FP64 direct-sum energies) over energy_steps steps
-    at the first theta.  fp64_walk: the same with the FP64 walk."""
+    at the first theta.  fp64_walk: the same with the FP64 walk; leaf,
+    group, walk_list: bucket size, targets per walk group, LDS list entries."""
     from next_amd.nbx import MODE_PHYSICS, PREC_FP64, PREC_MIXED, Engine
 
     walk_prec = PREC_FP64 if fp64_walk else PREC_MIXED
@@ -296,8 +300,8 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, 
     sub = np.random.Generator(np.random.PCG64(0)).choice(n, min(n, 20000), replace=False)
     rows, energy = [], None
     for j, theta in enumerate(thetas):
-        with Engine(mode=MODE_PHYSICS, theta=theta, eps=0.005, precision=walk_prec, leaf_size=16, group_size=64,
-                    adaptive_dt=False) as e:
+        with Engine(mode=MODE_PHYSICS, theta=theta, eps=0.005, precision=walk_prec, leaf_size=leaf, group_size=group,
+                    adaptive_dt=False, walk_list=walk_list) as e:
             e.upload(p)
             a = e.accel()
             d = np.linalg.norm(a_ref[sub], axis=1)
@@ -347,7 +351,8 @@ def physics(n: int, ref_ms: float, thetas=(0.32, 0.5), energy_steps: int = 200, 
                 energy = {"theta": theta, "steps": energy_steps, "E0": E0, "E1": E1,
                           "abs_dE_over_E": abs(E1 - E0) / abs(E0)}
         log("physics theta %g: %.3f ms/step, median force error %.2e" % (theta, ms, rows[-1]["median_rel_force_err"]))
-    return {"workload": f"cold uniform sphere N={n}, eps 0.005, fixed dt 5e-4, leaf 16, group 64, "
+    return {"workload": f"cold uniform sphere N={n}, eps 0.005, fixed dt 5e-4, leaf {leaf}, group {group}, "
+                        f"walk list {walk_list}, "
                         f"{'FP64' if fp64_walk else 'FP32'} walk / FP64 state",
             "by_theta": rows, "energy": energy}
 
@@ -500,6 +505,9 @@ def main() -> int:
                     help="run just this sample (for profiling) and print its JSON")
     ap.add_argument("--thetas", default="0.32,0.5", help="physics sample: opening angles (comma-separated)")
     ap.add_argument("--walk-precision", choices=["fp32", "fp64"], default="fp32", help="physics sample: walk precision")
+    ap.add_argument("--leaf", type=int, default=16, help="physics sample: bucket size")
+    ap.add_argument("--group", type=int, default=64, help="physics sample: targets per walk group")
+    ap.add_argument("--walk-list", type=int, default=128, help="physics sample: LDS list entries of the walk")
     ap.add_argument("--serve-config", action="store_true",
                     help="serving sample with the configuration's distributions (N up to 2^20, 50-500 steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -527,7 +535,8 @@ def main() -> int:
     if args.only:
         if args.only == "physics":
             res = physics(args.n, None, thetas=[float(t) for t in args.thetas.split(",")], energy_steps=0,
-                          fp64_walk=args.walk_precision == "fp64")
+                          fp64_walk=args.walk_precision == "fp64", leaf=args.leaf, group=args.group,
+                          walk_list=args.walk_list)
         elif args.only == "ensemble":
             res = ensemble(args.ensemble_systems, steps=16, warmup=1)
         elif args.only == "merger":

@@ -240,6 +240,20 @@ nbx_status nbx_energy_systems(nbx_ctx* ctx, int source, double* out);
  * the call leaves behind is the one it started from. */
 nbx_status nbx_autotune(nbx_ctx* ctx, const int32_t* leaf, const int32_t* group, int32_t ncand, double dt,
                         int32_t steps, double* ms_out);
+/* The same with the walk's list size as a third coordinate: candidate k is
+ * (leaf[k], group[k], list[k]) with list[k] as nbx_set_walk_list takes it
+ * (list may be null: the context's list size for every candidate). */
+nbx_status nbx_autotune_lists(nbx_ctx* ctx, const int32_t* leaf, const int32_t* group, const int32_t* list,
+                              int32_t ncand, double dt, int32_t steps, double* ms_out);
+
+/* Physics-mode walk setting (no counterpart in NEXT): entries of the
+ * per-wave LDS lists of accepted cells and bucket bodies that the walk
+ * collects before it evaluates them, 96, 128 (the default), 192 or 256.
+ * Longer lists evaluate more entries per flush with more lanes busy but hold
+ * more LDS per wave (fewer resident waves).  The FP32 flush sums change with
+ * it (the FP64 totals add the flushes).  NBX_ERR_ARG for another size or a
+ * reference-mode context. */
+nbx_status nbx_set_walk_list(nbx_ctx* ctx, int32_t entries);
 
 /* Captured runs (no counterpart in NEXT).  With on = 1, nbx_run replays its
  * steps from a HIP graph of a period of steps (2, or 2k in physics mode with

@@ -82,6 +82,7 @@ struct nbx_ctx {
   size_t hstage_bytes = 0;
   bool physics = false;       // NBX_MODE_PHYSICS
   double alpha = 0.0;         // physics mode: relative opening criterion (0: geometric only)
+  int walk_list = kPhysListDefault;   // physics mode: entries of the walk's LDS lists
   DBuf amag[2];               // |a| per body from the opening kick, in key order (FP32)
   int acur = 0;
   bool amag_live = false;     // amag[acur] follows the bodies through this step's sort
@@ -556,7 +557,8 @@ nbx_status walk(nbx_ctx* c, bool kick) {
     launch_phys_walk(P<Group>(c->tgroups), group_grid(c), P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                      c->cfg.leaf_size, P<double>(c->acc[0]), P<double>(c->acc[1]), P<double>(c->acc[2]), nullptr,
                      c->sc, kick, count, c->cfg.precision == NBX_PREC_FP64,
-                     c->amag_live ? P<float>(c->amag[c->acur]) : nullptr, c->alpha, c->cfg.theta, c->stream);
+                     c->amag_live ? P<float>(c->amag[c->acur]) : nullptr, c->alpha, c->cfg.theta, c->walk_list,
+                     c->stream);
   else
     launch_walk(P<Group>(c->groups), c->ngroups, P<RefNode>(c->nodes), P<WalkHead>(c->heads), view(c, c->cur),
                 sysa(c), c->cfg.theta,
@@ -1059,13 +1061,25 @@ uint64_t nbx_device_bytes(const nbx_ctx* c) {
   return total;
 }
 
+bool valid_walk_list(int32_t list) {
+  for (int v : kPhysLists)
+    if (v == list) return true;
+  return false;
+}
+
 nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, int32_t ncand, double dt,
                         int32_t steps, double* ms_out) {
+  return nbx_autotune_lists(c, leaf, group, nullptr, ncand, dt, steps, ms_out);
+}
+
+nbx_status nbx_autotune_lists(nbx_ctx* c, const int32_t* leaf, const int32_t* group, const int32_t* list,
+                              int32_t ncand, double dt, int32_t steps, double* ms_out) {
   NBXCHK(check_ctx(c));
   if (!c->physics) return fail(c, NBX_ERR_ARG, "the autotuner chooses physics-mode leaf and group sizes");
   if (ncand <= 0 || !leaf || !group || steps <= 0 || !(dt > 0.0)) return fail(c, NBX_ERR_ARG, "bad candidates");
   for (int32_t k = 0; k < ncand; ++k)
-    if (leaf[k] < 1 || leaf[k] > kPhysMaxLeaf || group[k] < 1 || group[k] > kPhysBlock)
+    if (leaf[k] < 1 || leaf[k] > kPhysMaxLeaf || group[k] < 1 || group[k] > kPhysBlock ||
+        (list && !valid_walk_list(list[k])))
       return fail(c, NBX_ERR_ARG, "candidate out of range");
   if (c->n == 0) return NBX_OK;
   // The state every trial starts from (and that the call leaves behind).
@@ -1092,12 +1106,14 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
     return NBX_OK;
   };
   const nbx_config keep = c->cfg;
+  const int keep_list = c->walk_list;
   const nbx_timing tim0 = c->tim;                          // trials do not count as steps
   int32_t best = -1;
   double best_ms = 0.0;
   for (int32_t k = 0; k < ncand; ++k) {
     c->cfg.leaf_size = leaf[k];
     c->cfg.group_size = group[k];
+    if (list) c->walk_list = list[k];
     ++c->gen;
     c->layout_valid = false;                               // groups of these sizes need a full build
     nbx_status st = nbx_run(c, dt, 1, nullptr);            // warm-up: buffers, first tree
@@ -1107,6 +1123,7 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
     const nbx_status rs = restore();
     if (st != NBX_OK || rs != NBX_OK) {
       c->cfg = keep;
+      c->walk_list = keep_list;
       c->tim = tim0;
       release_all();
       return st != NBX_OK ? st : rs;
@@ -1119,6 +1136,7 @@ nbx_status nbx_autotune(nbx_ctx* c, const int32_t* leaf, const int32_t* group, i
   c->cfg = keep;
   c->cfg.leaf_size = leaf[best];
   c->cfg.group_size = group[best];
+  c->walk_list = list ? list[best] : keep_list;
   ++c->gen;
   return NBX_OK;
 }
@@ -1128,6 +1146,16 @@ nbx_status nbx_set_alpha(nbx_ctx* c, double alpha) {
   if (!(alpha >= 0.0 && alpha < INFINITY)) return fail(c, NBX_ERR_ARG, "alpha must be finite and >= 0");
   if (alpha > 0.0 && !c->physics) return fail(c, NBX_ERR_ARG, "the relative opening criterion is a physics-mode option");
   c->alpha = alpha;
+  ++c->gen;
+  return NBX_OK;
+}
+
+nbx_status nbx_set_walk_list(nbx_ctx* c, int32_t entries) {
+  if (!c) return NBX_ERR_ARG;
+  if (!valid_walk_list(entries)) return fail(c, NBX_ERR_ARG, "the walk list holds 96, 128, 192 or 256 entries");
+  if (!c->physics) return fail(c, NBX_ERR_ARG, "the walk list size is a physics-mode setting");
+  if (entries != c->walk_list) c->acc_valid = false;   // forces of this setting from the next walk on
+  c->walk_list = entries;
   ++c->gen;
   return NBX_OK;
 }
@@ -1412,7 +1440,7 @@ static nbx_status tree_potential(nbx_ctx* c) {
   double* da = P<double>(c->dacc);
   launch_phys_walk(P<Group>(c->tgroups), group_grid(c), P<PhysNode>(c->pnodes), view(c, c->cur), sysa(c), c->cfg.eps,
                    c->cfg.leaf_size, da, da + n, da + 2 * n, P<double>(c->pot), c->sc, false, false,
-                   c->cfg.precision == NBX_PREC_FP64, nullptr, 0.0, c->cfg.theta, c->stream);
+                   c->cfg.precision == NBX_PREC_FP64, nullptr, 0.0, c->cfg.theta, c->walk_list, c->stream);
   HIPCHK(c, hipGetLastError());
   return NBX_OK;
 }

@@ -320,7 +320,7 @@ __device__ __forceinline__ double rsq<double>(double x) { return rsqrt(x); }
 // overflow.  All decisions are made in lockstep by one wave and compacted in
 // lane order, so the lists, and every sum, come out in the same order on
 // every run.
-template <typename T, bool KICK, bool POT, bool COUNT>
+template <typename T, int LIST, bool KICK, bool POT, bool COUNT>
 __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restrict__ groups,
                                                           const PhysNode* __restrict__ pn, StateArrays st,
                                                           SysArrays sy, double eps, int nleaf,
@@ -332,10 +332,10 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   __shared__ int32_t stk_e[kPhysStack];
   __shared__ int32_t bfirst[kWave];      // buckets of the current step: first body,
   __shared__ int32_t bstart[kWave];      //   offset in the step's flattened body list
-  __shared__ V4<T> ca[kPhysCells];       // cell: relative centre of mass, mass
-  __shared__ V4<T> cb[kPhysCells];       // cell: qxx qyy qxy qxz
-  __shared__ T cq[kPhysCells];           // cell: qyz
-  __shared__ V4<T> pl[kPhysParts];       // body: relative position, mass
+  __shared__ V4<T> ca[LIST];             // cell: relative centre of mass, mass
+  __shared__ V4<T> cb[LIST];             // cell: qxx qyy qxy qxz
+  __shared__ T cq[LIST];                 // cell: qyz
+  __shared__ V4<T> pl[LIST];             // body: relative position, mass
 
   const int lane = threadIdx.x;
   // The group count comes from the device (the compaction that wrote it ran
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       const bool busy = cur < end;
       if (!__ballot(busy)) break;
       if (COUNT) ++niter;
-      if (nc > kPhysCells - kWave) flush_cells();
+      if (nc > LIST - kWave) flush_cells();
 
       bool acc = false, bucket = false, donate = false;
       int32_t cnt = 0, first = 0, nskip = 0;
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       }
       __syncthreads();
       for (int base = 0; base < nbody; base += kWave) {
-        if (np > kPhysParts - kWave) flush_parts();
+        if (np > LIST - kWave) flush_parts();
         const int t = base + lane;
         if (t < nbody) {
           int lo_j = 0, hi_j = nb - 1;                       // last bucket with bstart <= t
@@ -661,13 +661,13 @@ __global__ __launch_bounds__(kBlock) void k_energy_final(const double* __restric
   for (int k = 0; k < 5; ++k) out[5 * s + k] = t[k];
 }
 
-template <typename T>
+template <typename T, int LIST>
 void walk_dispatch(unsigned nb, const Group* groups, const PhysNode* pn, const StateArrays& st, const SysArrays& sy,
                    double eps, int nleaf, double* ax, double* ay, double* az, double* pot, DevScalars* sc, bool kick,
                    bool count, const float* amag, float rel2, float th2, hipStream_t s) {
 #define NBX_PW(K, P, C)                                                                                          \
-  hipLaunchKernelGGL((k_phys_walk<T, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, groups, pn, st, sy, eps, nleaf, \
-                     ax, ay, az, pot, sc, amag, rel2, th2)
+  hipLaunchKernelGGL((k_phys_walk<T, LIST, K, P, C>), dim3(nb), dim3(kPhysBlock), 0, s, groups, pn, st, sy, eps, \
+                     nleaf, ax, ay, az, pot, sc, amag, rel2, th2)
   const bool p = pot != nullptr;
   if (kick) {
     if (count) NBX_PW(true, false, true); else NBX_PW(true, false, false);
@@ -677,6 +677,23 @@ void walk_dispatch(unsigned nb, const Group* groups, const PhysNode* pn, const S
     if (count) NBX_PW(false, false, true); else NBX_PW(false, false, false);
   }
 #undef NBX_PW
+}
+
+template <typename T>
+void walk_dispatch_list(int list, unsigned nb, const Group* groups, const PhysNode* pn, const StateArrays& st,
+                        const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
+                        DevScalars* sc, bool kick, bool count, const float* amag, float rel2, float th2,
+                        hipStream_t s) {
+  switch (list) {
+    case 96: walk_dispatch<T, 96>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
+                                  th2, s); break;
+    case 192: walk_dispatch<T, 192>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag,
+                                    rel2, th2, s); break;
+    case 256: walk_dispatch<T, 256>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag,
+                                    rel2, th2, s); break;
+    default: walk_dispatch<T, 128>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag,
+                                   rel2, th2, s); break;
+  }
 }
 
 }  // namespace
@@ -742,15 +759,17 @@ void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const
 void launch_phys_walk(const Group* groups, int64_t grid_groups, const PhysNode* pn, const StateArrays& st,
                       const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
                       DevScalars* sc, bool kick, bool count, bool fp64, const float* amag, double alpha,
-                      double theta, hipStream_t s) {
+                      double theta, int list, hipStream_t s) {
   if (grid_groups <= 0) return;
   const unsigned nb = (unsigned)(div_up(grid_groups, 8) * 8);
   const float rel2 = (float)(alpha * alpha), th2 = (float)(theta * theta);
   if (alpha <= 0.0) amag = nullptr;
   if (fp64)
-    walk_dispatch<double>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2, th2, s);
+    walk_dispatch_list<double>(list, nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
+                               th2, s);
   else
-    walk_dispatch<float>(nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2, th2, s);
+    walk_dispatch_list<float>(list, nb, groups, pn, st, sy, eps, nleaf, ax, ay, az, pot, sc, kick, count, amag, rel2,
+                              th2, s);
 }
 
 void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,

@@ -37,8 +37,11 @@ __host__ __device__ inline double leaf_mass(const PhysNode& nd) {
 constexpr int kPhysBlock = 64;      // one wave = one group of targets
 // Per-wave LDS (7.4 KB with these sizes) bounds the walk's occupancy at
 // about 5 waves per SIMD; a 256-entry stack and body list (11 KB) allowed 3.
-constexpr int kPhysCells = 128;     // per-wave LDS cell list
-constexpr int kPhysParts = 128;     // per-wave LDS body list
+// Entries of the per-wave LDS lists (accepted cells, bodies of accepted
+// buckets) a walk collects before it evaluates them: a walk setting
+// (nbx_set_walk_list), one of kPhysLists.
+constexpr int kPhysListDefault = 128;
+constexpr int kPhysLists[] = {96, 128, 192, 256};
 constexpr int kPhysMaxLeaf = 64;    // bucket size limit (one body per lane)
 constexpr int kPhysStack = 32;      // per-wave LDS stack of donated node ranges
 constexpr int kPhysAuxBytes = 128;  // FP64 moments per node (mass, bmax, Q, box)
@@ -84,7 +87,7 @@ void launch_group_compact(int64_t n, const int32_t* gcnt, uint32_t* gmark, const
 void launch_phys_walk(const Group* groups, int64_t grid_groups, const PhysNode* pn, const StateArrays& st,
                       const SysArrays& sy, double eps, int nleaf, double* ax, double* ay, double* az, double* pot,
                       DevScalars* sc, bool kick, bool count, bool fp64, const float* amag, double alpha,
-                      double theta, hipStream_t s);
+                      double theta, int list, hipStream_t s);
 // FP64 direct sum with Plummer softening over all pairs of each system (G = 1).
 void launch_direct64(int64_t n, const StateArrays& st, const SysArrays& sy, double eps, double* ax, double* ay,
                      double* az, double* pot, hipStream_t s);

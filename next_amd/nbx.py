@@ -23,6 +23,7 @@ MODE_REFERENCE, MODE_PHYSICS = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTS = ["nbx_config_default", "nbx_create", "nbx_destroy", "nbx_last_error", "nbx_set_theta", "nbx_set_alpha",
+           "nbx_set_walk_list", "nbx_autotune_lists",
            "nbx_set_rebuild_interval", "nbx_trim", "nbx_device_bytes",
            "nbx_upload",
            "nbx_download", "nbx_size", "nbx_adaptive_dt", "nbx_step", "nbx_run", "nbx_accel", "nbx_get_timing",
@@ -94,6 +95,7 @@ def lib():
     L.nbx_last_error.restype = C.c_char_p
     L.nbx_set_theta.argtypes = [VP, C.c_double]
     L.nbx_set_alpha.argtypes = [VP, C.c_double]
+    L.nbx_set_walk_list.argtypes = [VP, C.c_int32]
     L.nbx_set_rebuild_interval.argtypes = [VP, C.c_int32]
     L.nbx_set_graph.argtypes = [VP, C.c_int]
     L.nbx_trim.argtypes = [VP]
@@ -116,6 +118,7 @@ def lib():
     L.nbx_debug_groups.argtypes = [VP, C.c_int64, i32p, i32p, i32p, C.POINTER(C.c_int64)]
     L.nbx_debug_order.argtypes = [VP, i32p, i32p]
     L.nbx_autotune.argtypes = [VP, i32p, i32p, C.c_int32, C.c_double, C.c_int32, f64p]
+    L.nbx_autotune_lists.argtypes = [VP, i32p, i32p, i32p, C.c_int32, C.c_double, C.c_int32, f64p]
     L.nbx_potential.argtypes = [VP, f64p]
     L.nbx_direct.argtypes = [VP, VP, VP, VP, VP]
     L.nbx_energy.argtypes = [VP, C.c_int, f64p]
@@ -161,7 +164,8 @@ class Engine:
 
     def __init__(self, theta: float = 0.5, device: int = 0, mode: int = MODE_REFERENCE,
                  precision: int = PREC_FP64, eps: float = 0.01, leaf_size: int = 16, group_size: int = 64,
-                 adaptive_dt: bool = True, alpha: float = 0.0, rebuild_every: int = 1, graph: int = 0):
+                 adaptive_dt: bool = True, alpha: float = 0.0, rebuild_every: int = 1, graph: int = 0,
+                 walk_list: int = 0):
         L = lib()
         cfg = Config()
         L.nbx_config_default(C.byref(cfg))
@@ -178,6 +182,8 @@ class Engine:
             self.set_alpha(alpha)
         if rebuild_every != 1:
             self.set_rebuild_interval(rebuild_every)
+        if walk_list:
+            self.set_walk_list(walk_list)
         if graph:
             self.set_graph(graph)
 
@@ -210,6 +216,10 @@ class Engine:
     def set_alpha(self, alpha: float):
         """Physics mode: relative opening criterion (0 turns it off)."""
         self._check(lib().nbx_set_alpha(self._h, alpha))
+
+    def set_walk_list(self, entries: int):
+        """Physics mode: entries of the walk's LDS lists (96, 128, 192 or 256)."""
+        self._check(lib().nbx_set_walk_list(self._h, entries))
 
     def set_rebuild_interval(self, k: int):
         """Physics mode: full tree build every k-th step, refits between."""
@@ -356,11 +366,17 @@ class Engine:
         return ids[: self.n], sys_[: self.n]
 
     def autotune(self, candidates, dt: float, steps: int = 2):
-        """Time each (leaf_size, group_size) candidate on the current systems
-        (physics mode), keep the fastest; returns (best, ms per step of each)."""
+        """Time each (leaf_size, group_size) or (leaf_size, group_size,
+        walk_list) candidate on the current systems (physics mode), keep the
+        fastest; returns (best, ms per step of each)."""
         leaf = np.array([c[0] for c in candidates], np.int32)
         group = np.array([c[1] for c in candidates], np.int32)
         ms = np.zeros(len(candidates))
+        if any(len(c) > 2 for c in candidates):
+            lst = np.array([c[2] for c in candidates], np.int32)
+            self._check(lib().nbx_autotune_lists(self._h, leaf, group, lst, len(candidates), dt, steps, ms))
+            best = int(np.argmin(ms))
+            return (int(leaf[best]), int(group[best]), int(lst[best])), ms
         self._check(lib().nbx_autotune(self._h, leaf, group, len(candidates), dt, steps, ms))
         best = int(np.argmin(ms))
         return (int(leaf[best]), int(group[best])), ms

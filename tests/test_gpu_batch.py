@@ -135,6 +135,40 @@ def test_autotune_restores_state_and_applies_choice():
         assert np.array_equal(e.download(), after)
 
 
+def test_autotune_lists_restores_state_and_applies_choice():
+    """nbx_autotune_lists times (leaf, group, walk list) candidates the same
+    way: the state is left as found, and the steps that follow equal a run
+    made with the chosen leaf, group and list sizes from the start."""
+    sy = [ic.plummer(n, seed=50 + i) for i, n in enumerate((5000, 2500))]
+    cands = [(16, 64, 128), (16, 64, 256), (16, 64, 96), (32, 64, 192)]
+    with make("physics", False) as e:
+        e.upload_batch(sy)
+        before = e.download()
+        best, ms = e.autotune(cands, 1e-3, steps=2)
+        assert np.array_equal(e.download(), before)
+        assert best in cands and len(ms) == len(cands) and np.all(ms > 0)
+        e.run(1e-3, 3)
+        after = e.download()
+    with Engine(theta=0.5, mode=MODE_PHYSICS, precision=PREC_MIXED, eps=0.01, leaf_size=best[0],
+                group_size=best[1], adaptive_dt=False) as e:
+        e.set_walk_list(best[2])
+        e.upload_batch(sy)
+        e.run(1e-3, 3)
+        assert np.array_equal(e.download(), after)
+
+
+def test_walk_list_setting_checks():
+    """nbx_set_walk_list takes 96, 128, 192 or 256 entries, in physics mode only."""
+    with make("physics", False) as e:
+        for bad in (0, 64, 100, 512):
+            with pytest.raises(NbxError):
+                e.set_walk_list(bad)
+        e.set_walk_list(192)
+    with make("reference", False) as e:
+        with pytest.raises(NbxError):
+            e.set_walk_list(128)
+
+
 @pytest.mark.parametrize("mode", ["reference", "physics"])
 def test_reused_context_matches_fresh(mode):
     """A context whose buffers were sized by earlier, smaller and larger

@@ -75,6 +75,30 @@ def test_group_walk_force_error(precision, theta, med_tol, p99_tol):
     assert med <= med_tol and p99 <= p99_tol
 
 
+@pytest.mark.parametrize("precision", [PREC_MIXED, PREC_FP64])
+def test_walk_list_sizes_agree(precision):
+    """The walk's LDS list size changes only how the accepted cells and
+    bodies are batched: every size gives the default's force error against
+    the FP64 direct sum, and forces that differ from the default's by FP32
+    (FP64) rounding of the per-flush sums only."""
+    p = ic.plummer(32768, seed=33)
+    got = {}
+    with phys(theta=0.5, eps=0.01, precision=precision) as e:
+        e.upload(p)
+        ad, _ = e.direct()
+        for lst in (96, 128, 192, 256):
+            e.set_walk_list(lst)
+            got[lst] = e.accel()
+    base = got[128]
+    med0 = float(np.median(rel_err(base, ad)))
+    tol = 1e-5 if precision == PREC_MIXED else 1e-12
+    for lst, a in got.items():
+        med = float(np.median(rel_err(a, ad)))
+        diff = float(np.max(rel_err(a, base)))
+        print(f"list {lst}: median {med:.3e} (128: {med0:.3e}), max diff to 128 {diff:.2e}")
+        assert abs(med - med0) <= 0.02 * med0 and diff <= tol
+
+
 def test_energy_and_momentum_over_steps():
     p = ic.plummer(16384, seed=1)
     with phys(theta=0.5, eps=0.01, precision=PREC_MIXED) as e:
