@@ -182,14 +182,20 @@ def ensemble(nsys: int, steps: int, warmup: int, ref_body_steps_per_s: float = N
                 rebuild_every=REBUILD_EVERY) as e:
         e.upload_batch(systems)
         best, trial_ms = e.autotune(cands, 1e-3, steps=4)   # leaves the state as it was
-        e.run(1e-3, warmup)
+        # warm-up for both legs: one captured period after the steps up to the
+        # next rebuild, so the graph leg captures before its timed run, which
+        # then starts at a rebuild step and replays whole periods
+        warm = max(warmup, REBUILD_EVERY - 1 + 2 * REBUILD_EVERY)
+        e.run(1e-3, warm)
         t0 = time.perf_counter()
         e.run(1e-3, steps)
         dt_eager = time.perf_counter() - t0
         tim_eager = e.timing()
-        # the same steps replayed from a captured graph (nbx_set_graph)
+        # the same steps, from the same start, replayed from a captured graph
+        # (nbx_set_graph; the systems evolve, so later steps would cost more)
+        e.upload_batch(systems)
         e.set_graph(1)
-        e.run(1e-3, 2 * REBUILD_EVERY)                     # capture
+        e.run(1e-3, warm)                                  # first forces and the capture
         e.reset_timing()
         t0 = time.perf_counter()
         e.run(1e-3, steps)
@@ -559,12 +565,15 @@ def main() -> int:
     t0 = time.perf_counter()
     eng.run(args.base_dt, args.steps)   # synchronous: returns after the stream drains
     wall_ms = (time.perf_counter() - t0) * 1e3
-    # beside it: K steps replayed from a captured graph (nbx_set_graph; bit for
-    # bit the same arithmetic, one host wait per run)
+    # beside it: the same K steps, from the same start, replayed from a
+    # captured graph (nbx_set_graph; bit for bit the same arithmetic, one host
+    # wait per run).  The collapse makes later steps dearer, so every leg
+    # starts from the uploaded state.
     captured_ms, tg = None, None
     if not args.no_graph:
+        eng.upload(p)
         eng.set_graph(1)
-        eng.run(args.base_dt, 2)        # capture
+        eng.run(args.base_dt, max(args.warmup, 2))   # first forces and the capture
         eng.reset_timing()
         t0 = time.perf_counter()
         eng.run(args.base_dt, args.steps)
@@ -572,6 +581,8 @@ def main() -> int:
         tg = eng.timing()
         eng.set_graph(0)
 
+    eng.upload(p)
+    eng.run(args.base_dt, args.warmup)
     eng.set_profiling(1)      # HIP events around each phase, on the engine's stream (one by one)
     eng.reset_timing()
     eng.run(args.base_dt, args.steps)

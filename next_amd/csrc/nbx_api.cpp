@@ -606,6 +606,7 @@ nbx_status ensure_accel(nbx_ctx* c) {
   if (c->acc_valid) return NBX_OK;
   record(c, EV_START);
   NBXCHK(build_tree(c));
+  c->since_build = 0;                   // the refit schedule counts from this build
   NBXCHK(walk(c, false));
   return NBX_OK;
 }

@@ -153,6 +153,30 @@ def test_large_tree_vs_oracle():
         assert np.array_equal(t[k], o[k]), k
 
 
+def test_tree_tracks_a_deepening_collapse():
+    """Each build sorts only the key digits the previous build's depth needs
+    and orders the runs of bodies that agree on them in place; while a cold
+    sphere collapses and its tree deepens from step to step, every step's
+    tree must still be the oracle's insertion tree of the same positions,
+    bit for bit."""
+    p = ic.uniform_sphere(32768, seed=9)
+    depths = []
+    with Engine(theta=0.5) as e:
+        e.upload(p)
+        for steps in (1, 4, 10, 25):
+            e.run(2e-2, steps)
+            x, y, z, vx, vy, vz = e.download()
+            t = e.debug_tree()
+            o = orc.tree(x, y, z, p.m)
+            for k in ("depth", "leaf", "body", "cmask", "perm"):
+                assert np.array_equal(t[k], o[k]), (steps, k)
+            for k in ("geo", "mass", "com", "q"):
+                assert np.array_equal(t[k], o[k]), (steps, k)
+            depths.append(int(t["depth"].max()))
+    print("tree depths", depths)
+    assert depths[-1] >= depths[0] + 2      # the tree deepened: the runs of the truncated sorts were fixed
+
+
 def test_million_body_keys_and_sampled_accel():
     """The headline workload size: 1,048,576-body cold sphere.  Keys are
     bit-exact for every body; accelerations are checked on 4,096 sampled
