@@ -465,21 +465,46 @@ def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(2
                         "200 GB budget"}
 
 
-# Algorithmic HBM bytes per body of the streaming phases (state: x y z vx vy
+# Algorithmic HBM bytes per body of the step's phases (state: x y z vx vy
 # vz m as FP64, type id sys as int32; keys 8 B, sort values 4 B):
 #   kick + drift: read x..vz, a, sys (76 B), write x..vz (48 B);
 #   bbox + keys: bbox reads x y z (24 B); keys read x y z sys (28 B) and write
 #     key and index (12 B);
-#   sort + gather: 8 radix passes, each reading the keys for the histogram
-#     (8 B) and keys + values for the scatter (12 B) and writing them (12 B);
-#     then the gather reads the permutation (4 B) and the 68-byte state and
-#     writes it (68 B).
-PHASE_BYTES = {"kick_ms": 124, "keys_ms": 64, "sort_ms": 8 * 32 + 140}
+#   sort + gather: the digit count reads the keys (8 B); each of the P radix
+#     passes reads keys and values (12 B) and writes them (12 B); the gather
+#     reads the permutation (4 B) and the 68-byte state and writes it (68 B).
+#     P = 3 * depth / 8 rounded up, at most 8 (the sort covers the digits
+#     the last build's depth needs);
+#   node layout: lcp reads key and system (12 B) and writes lcp and node
+#     count (8 B); the count's scan reads it twice and writes it (12 B); the
+#     layout kernel reads lcp, preorder offset, x y z m and system (44 B) and
+#     writes the leaf's 80-byte record and 32-byte walk head (112 B), plus
+#     two 4-byte list entries per internal node;
+#   upward pass: per internal node the two list entries (8 B), the first
+#     body's system (4 B) and the record and head it writes (112 + 32 B); per
+#     node but the roots, the 64-byte sector of its record its parent reads.
+HBM_FIXED_BYTES = {"kick_ms": 124, "keys_ms": 64}
+
+
+def phase_bytes(n: int, nodes: int, depth: int) -> dict:
+    """Bytes per body of each phase (the comment above) for a tree of
+    `nodes` nodes (leaves included) and `depth` levels over n bodies."""
+    passes = min(8, -(-3 * depth // 8)) if 0 < depth < 21 else 8
+    internal = max(nodes - n, 0)
+    per_int = internal / n if n else 0.0
+    out = dict(HBM_FIXED_BYTES)
+    out["sort_ms"] = 8 + 24 * passes + 140
+    out["build_ms"] = round(12 + 8 + 12 + 44 + 112 + 8 * per_int, 1)
+    out["upward_ms"] = round(per_int * (8 + 4 + 144) + max(nodes - 1, 0) / n * 64, 1) if n else 0.0
+    return out
 
 
 def phase_rooflines(n: int, tim: dict, k: int) -> dict:
+    per = phase_bytes(n, int(tim.get("nodes", 0)), int(tim.get("depth", 0)))
     out = {}
-    for key, per_body in PHASE_BYTES.items():
+    for key, per_body in per.items():
+        if key not in tim:
+            continue
         ms = tim[key] / k
         gbs = per_body * n / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         out[key[:-3]] = {"bytes_per_body": per_body, "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}

@@ -69,3 +69,18 @@ def test_phase_rates():
     gbs = 124 * (1 << 20) / 0.03e-3 / 1e9
     assert out["kick"]["GB/s"] == pytest.approx(gbs, abs=0.05)
     assert out["kick"]["frac"] == pytest.approx(gbs / 8000.0, abs=1e-4)
+
+
+def test_phase_bytes_follow_the_tree():
+    """Sort bytes follow the passes the depth needs; layout and upward bytes
+    the internal nodes per body (1.55 M nodes over 2^20 bodies)."""
+    n = 1 << 20
+    b16 = bench.phase_bytes(n, 1549025, 16)
+    b20 = bench.phase_bytes(n, 1549025, 20)
+    assert b16["sort_ms"] == 8 + 24 * 6 + 140 and b20["sort_ms"] == 8 + 24 * 8 + 140
+    per_int = (1549025 - n) / n
+    assert b16["build_ms"] == pytest.approx(188 + 8 * per_int, abs=0.1)
+    assert b16["upward_ms"] == pytest.approx(per_int * 156 + (1549025 - 1) / n * 64, abs=0.1)
+    out = bench.phase_rooflines(n, {"build_ms": 1.0, "upward_ms": 1.7, "nodes": 1549025, "depth": 16}, 10)
+    assert set(out) == {"build", "upward"}
+    assert out["build"]["GB/s"] == pytest.approx(b16["build_ms"] * n / 0.1e-3 / 1e9, abs=0.1)
