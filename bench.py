@@ -576,7 +576,8 @@ def main() -> int:
         elif args.only == "merger":
             res = merger()
         else:
-            res = [serving(args.serve_requests, 50.0)]
+            full = dict(n_max=1 << 20, steps=(50, 500)) if args.serve_config else {}
+            res = [serving(args.serve_requests, rate, **full) for rate in (50.0, 200.0)]
         print(json.dumps(res))
         return 0
 
