@@ -291,6 +291,10 @@ template <typename T>
 struct alignas(4 * sizeof(T)) V4 {
   T x, y, z, w;
 };
+template <typename T>
+struct alignas(2 * sizeof(T)) V2 {
+  T x, y;
+};
 
 // Sum over the lanes that share a target slot (lanes slot + p * width).
 template <typename T>
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
   __shared__ int32_t bstart[kWave];      //   offset in the step's flattened body list
   __shared__ V4<T> ca[LIST];             // cell: relative centre of mass, mass
   __shared__ V4<T> cb[LIST];             // cell: qxx qyy qxy qxz
-  __shared__ T cq[LIST];                 // cell: qyz
+  __shared__ V2<T> cq[LIST];             // cell: qyz qzz
   __shared__ V4<T> pl[LIST];             // body: relative position, mass
 
   const int lane = threadIdx.x;
@@ -398,32 +402,55 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
     uint32_t ncell_tot = 0, npart_tot = 0, niter = 0, nopen = 0;
     int nc = 0, np = 0;
 
+    // The terms of list entry k added to the running sums, every product
+    // fused into the sum it feeds: a = m d / r^3 + 2.5 (d.Q.d) d / r^7 - Q d / r^5
+    // and phi = -m / r - 0.5 (d.Q.d) / r^5, d from the target to the cell.
+    auto cell_term = [&](int k, T& fx, T& fy, T& fz, T& fp) {
+      const V4<T> a = ca[k];
+      const V4<T> b = cb[k];
+      const V2<T> q = cq[k];
+      const T dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
+      const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, e2)));
+      const T ri = rsq<T>(r2);
+      const T ri2 = ri * ri;
+      const T ri3 = ri2 * ri;
+      const T ri5 = ri3 * ri2;
+      const T qdx = fma(b.w, dz, fma(b.z, dy, b.x * dx));
+      const T qdy = fma(q.x, dz, fma(b.y, dy, b.z * dx));
+      const T qdz = fma(q.y, dz, fma(q.x, dy, b.w * dx));
+      const T dqd = fma(dz, qdz, fma(dy, qdy, dx * qdx));
+      const T u = dqd * ri5;
+      const T s = fma((T)2.5 * ri2, u, a.w * ri3);
+      fx = fma(-ri5, qdx, fma(s, dx, fx));
+      fy = fma(-ri5, qdy, fma(s, dy, fy));
+      fz = fma(-ri5, qdz, fma(s, dz, fz));
+      if (POT) fp = fma((T)-0.5, u, fma(-a.w, ri, fp));
+    };
+    auto body_term = [&](int k, T& fx, T& fy, T& fz, T& fp) {
+      const V4<T> b = pl[k];
+      const T dx = b.x - tx, dy = b.y - ty, dz = b.z - tz;
+      const T r2 = fma(dz, dz, fma(dy, dy, fma(dx, dx, e2)));
+      const T ri = rsq<T>(r2);
+      const T mri = b.w * ri;
+      const T mr3 = mri * (ri * ri);
+      fx = fma(mr3, dx, fx);
+      fy = fma(mr3, dy, fy);
+      fz = fma(mr3, dz, fz);
+      if (POT) fp -= mri;
+    };
+    // A group of more than 32 targets has one lane per target, and every
+    // lane takes every entry: the loop is wave-uniform (scalar control, one
+    // LDS address for all lanes).
     auto flush_cells = [&]() {
       __syncthreads();
       T fx = 0, fy = 0, fz = 0, fp = 0;
-      for (int k = part; k < nc; k += parts) {
-        const V4<T> a = ca[k];
-        const V4<T> b = cb[k];
-        const T qyz = cq[k];
-        const T dx = a.x - tx, dy = a.y - ty, dz = a.z - tz;
-        const T r2 = dx * dx + dy * dy + dz * dz + e2;
-        const T ri = rsq<T>(r2);
-        const T ri2 = ri * ri;
-        const T ri3 = ri2 * ri;
-        const T ri5 = ri3 * ri2;
-        const T qzz = -(b.x + b.y);
-        const T qdx = b.x * dx + b.z * dy + b.w * dz;
-        const T qdy = b.z * dx + b.y * dy + qyz * dz;
-        const T qdz = b.w * dx + qyz * dy + qzz * dz;
-        const T dqd = dx * qdx + dy * qdy + dz * qdz;
-        const T mono = a.w * ri3;
-        const T c7 = (T)2.5 * dqd * ri5 * ri2;
-        fx += (mono + c7) * dx - ri5 * qdx;
-        fy += (mono + c7) * dy - ri5 * qdy;
-        fz += (mono + c7) * dz - ri5 * qdz;
-        if (POT) fp -= a.w * ri + (T)0.5 * dqd * ri5;
+      if (parts == 1) {
+#pragma unroll 2
+        for (int k = 0; k < nc; ++k) cell_term(k, fx, fy, fz, fp);
+      } else {
+        for (int k = part; k < nc; k += parts) cell_term(k, fx, fy, fz, fp);
+        combine_parts(fx, fy, fz, fp, width);
       }
-      combine_parts(fx, fy, fz, fp, width);
       Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
       if (POT) P += (double)fp;
       if (COUNT) ncell_tot += nc;
@@ -433,18 +460,13 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
     auto flush_parts = [&]() {
       __syncthreads();
       T fx = 0, fy = 0, fz = 0, fp = 0;
-      for (int k = part; k < np; k += parts) {
-        const V4<T> b = pl[k];
-        const T dx = b.x - tx, dy = b.y - ty, dz = b.z - tz;
-        const T r2 = dx * dx + dy * dy + dz * dz + e2;
-        const T ri = rsq<T>(r2);
-        const T mr3 = b.w * ri * ri * ri;
-        fx += mr3 * dx;
-        fy += mr3 * dy;
-        fz += mr3 * dz;
-        if (POT) fp -= b.w * ri;
+      if (parts == 1) {
+#pragma unroll 2
+        for (int k = 0; k < np; ++k) body_term(k, fx, fy, fz, fp);
+      } else {
+        for (int k = part; k < np; k += parts) body_term(k, fx, fy, fz, fp);
+        combine_parts(fx, fy, fz, fp, width);
       }
-      combine_parts(fx, fy, fz, fp, width);
       Ax += (double)fx; Ay += (double)fy; Az += (double)fz;
       if (POT) P += (double)fp;
       if (COUNT) npart_tot += np;
@@ -502,7 +524,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
         const PhysNode& nd = pn[cur];
         ca[pos] = V4<T>{(T)(ncx - gcx), (T)(ncy - gcy), (T)(ncz - gcz), (T)nd.m};
         cb[pos] = V4<T>{(T)nd.qxx, (T)nd.qyy, (T)nd.qxy, (T)nd.qxz};
-        cq[pos] = (T)nd.qyz;
+        cq[pos] = V2<T>{(T)nd.qyz, (T)(-((double)nd.qxx + (double)nd.qyy))};
       }
       nc += (int)__popcll(am);
       // opened nodes hand [skip, end) to the stack while it has room
@@ -519,7 +541,7 @@ __global__ __launch_bounds__(kPhysBlock) void k_phys_walk(const Group* __restric
       // buckets -> body list, all lanes loading bodies in parallel
       const uint32_t cb_n = bucket ? (uint32_t)cnt : 0u;
       const uint32_t bincl = wave_incl_scan_u32(cb_n);
-      const int nbody = (int)__shfl(bincl, kWave - 1, kWave);
+      const int nbody = __builtin_amdgcn_readlane((int)bincl, kWave - 1);   // wave-uniform
       const uint64_t bm = __ballot(bucket);
       const int nb = (int)__popcll(bm);
       if (bucket) {
