@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 namespace nbx {
 namespace {
@@ -419,6 +420,12 @@ __global__ __launch_bounds__(kBlock) void k_emit(int64_t n, const int32_t* __res
   place_internal_nodes(valid, lp, li, pre, skipv, part, sc, level_list, end_list,
                        cap_int);
   if (!valid) return;
+  // One all-zero record after the last node: the walk's evaluation reads it
+  // for a lane with no entry left and for a leaf's quadrupole.
+  if (i == n - 1) {
+    const uint32_t z = (uint32_t)sc->total_nodes;
+    if (z < cap_nodes) nodes[z] = RefNode{};
+  }
   // Internal nodes are written by the upward pass (their records, skip
   // pointers included, come from the node lists placed above).
   const int ns = li > lp ? li - lp : 0;
@@ -719,61 +726,66 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   uint32_t npp = 0, npc = 0, nop = 0, nit = 0, nev = 0;
   int head = 0, len = 0;                                   // this lane's ring: [head, head + len)
+  const uint32_t zoff = (uint32_t)sc->total_nodes << 7;   // the all-zero record (k_emit)
 
   // Evaluate up to `limit` of the oldest entries (octree.h:139-176 for an
   // accepted node); an entry is the node index, with bit 31 set for a leaf
-  // (Q = 0).
-  auto evaluate = [&](int limit) {
+  // (Q = 0).  DM: some target of the wave is dark matter (its softening
+  // floor, octree.h:146-148, is a call); without one the loop is straight
+  // code.
+  auto evaluate = [&](auto dm, int limit) {
     const int take = len < limit ? len : limit;
     for (int k = 0; __builtin_amdgcn_ballot_w64(k < take); ++k) {
       if (COUNT) ++nev;
-      if (k < take) {
-        int slot = head + k;
-        if (slot >= kWalkList) slot -= kWalkList;
-        const int32_t e = mine[slot * kWave];
-        const bool leaf = e < 0;
-        const RefNode* nd = nodes + (e & 0x7fffffff);
-        const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
-        const double m = nd->m;
-        // octree.h:139-172 with products fused into FMAs and the square
-        // roots and divides faithfully rounded (a few ulp per interaction):
-        // r2 = |d|^2, dist = sqrt(r2 + 1e-20), eps from softening.h:26-36,
-        // dinv = 1/sqrt(r2 + eps^2).
-        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        const double r2e = r2 + 1e-20;
-        const double dist = r2e * rsq_nr(r2e);
-        double eps = nd->esum * rcp_nr(__builtin_fma(dist, 10.0, 1.0));
-        if (eps < 1e-4) eps = 1e-4;
-        if (pt == 1) eps = dm_softening(eps, nd->size, m, pm);   // octree.h:146-148
-        const double dinv = rsq_nr(__builtin_fma(eps, eps, r2));
-        const double inv2 = dinv * dinv;
-        const double inv3 = inv2 * dinv;
-        double c = m * inv3;                                   // G = 1
-        if (!leaf) {                                           // leaves carry Q = 0
-          // quadrupole term 0.5 (2 (Q d) inv5 - 5 q inv7 d), q = d.Q.d
-          // (octree.h:159-172), as (Q d) inv5 + d (-(2.5 q) inv7)
-          const double qxx = nd->qxx, qyy = nd->qyy, qzz = nd->qzz;
-          const double qxy = nd->qxy, qxz = nd->qxz, qyz = nd->qyz;
-          const double inv5 = inv3 * inv2;
-          const double inv7 = inv5 * inv2;
-          const double sx = __builtin_fma(qxz, dz, __builtin_fma(qxy, dy, qxx * dx));
-          const double sy = __builtin_fma(qyz, dz, __builtin_fma(qyy, dy, qxy * dx));
-          const double sz = __builtin_fma(qzz, dz, __builtin_fma(qyz, dy, qxz * dx));
-          const double q = __builtin_fma(dz, sz, __builtin_fma(dy, sy, dx * sx));
-          c = __builtin_fma(-2.5 * q, inv7, c);
-          a0 = __builtin_fma(sx, inv5, a0);
-          a1 = __builtin_fma(sy, inv5, a1);
-          a2 = __builtin_fma(sz, inv5, a2);
-        }
-        a0 = __builtin_fma(dx, c, a0);
-        a1 = __builtin_fma(dy, c, a1);
-        a2 = __builtin_fma(dz, c, a2);
-      }
+      int slot = head + k;
+      if (slot >= kWalkList) slot -= kWalkList;
+      // Every lane runs every round: a lane with no entry left reads the
+      // all-zero record after the last node, and so does a leaf entry for
+      // its quadrupole (leaf records carry none); both add exact zeros
+      // (m = 0, Q = 0, every factor finite).  Byte offsets: e << 7 drops
+      // the leaf bit.
+      const int32_t e = mine[slot * kWave];
+      const bool on = k < take;
+      const uint32_t off = on ? (uint32_t)e << 7 : zoff;
+      const uint32_t qoff = (on && e >= 0) ? off : zoff;
+      const RefNode* nd = reinterpret_cast<const RefNode*>(reinterpret_cast<const char*>(nodes) + off);
+      const RefNode* qn = reinterpret_cast<const RefNode*>(reinterpret_cast<const char*>(nodes) + qoff);
+      const double dx = nd->cx - px, dy = nd->cy - py, dz = nd->cz - pz;
+      const double m = nd->m;
+      // octree.h:139-172 with products fused into FMAs and the square
+      // roots and divides faithfully rounded (a few ulp per interaction):
+      // r2 = |d|^2, dist = sqrt(r2 + 1e-20), eps from softening.h:26-36,
+      // dinv = 1/sqrt(r2 + eps^2).
+      const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      const double r2e = r2 + 1e-20;
+      const double dist = r2e * rsq_nr(r2e);
+      double eps = nd->esum * rcp_nr(__builtin_fma(dist, 10.0, 1.0));
+      if (eps < 1e-4) eps = 1e-4;
+      if (decltype(dm)::value && pt == 1) eps = dm_softening(eps, nd->size, m, pm);   // octree.h:146-148
+      const double dinv = rsq_nr(__builtin_fma(eps, eps, r2));
+      const double inv2 = dinv * dinv;
+      const double inv3 = inv2 * dinv;
+      const double inv5 = inv3 * inv2;
+      const double inv7 = inv5 * inv2;
+      // quadrupole term 0.5 (2 (Q d) inv5 - 5 q inv7 d), q = d.Q.d
+      // (octree.h:159-172), as (Q d) inv5 + d (-(2.5 q) inv7)
+      const double qxx = qn->qxx, qyy = qn->qyy, qzz = qn->qzz;
+      const double qxy = qn->qxy, qxz = qn->qxz, qyz = qn->qyz;
+      const double sx = __builtin_fma(qxz, dz, __builtin_fma(qxy, dy, qxx * dx));
+      const double sy = __builtin_fma(qyz, dz, __builtin_fma(qyy, dy, qxy * dx));
+      const double sz = __builtin_fma(qzz, dz, __builtin_fma(qyz, dy, qxz * dx));
+      const double q = __builtin_fma(dz, sz, __builtin_fma(dy, sy, dx * sx));
+      const double c = __builtin_fma(-2.5 * q, inv7, m * inv3);   // G = 1
+      a0 = __builtin_fma(dx, c, __builtin_fma(sx, inv5, a0));
+      a1 = __builtin_fma(dy, c, __builtin_fma(sy, inv5, a1));
+      a2 = __builtin_fma(dz, c, __builtin_fma(sz, inv5, a2));
     }
     head += take;
     if (head >= kWalkList) head -= kWalkList;
     len -= take;
   };
+  // dark-matter targets in this wave (wave-uniform)
+  const bool any_dm = __builtin_amdgcn_ballot_w64(pt == 1) != 0;
 
   const int32_t root = sy.sroot[g.sys];
   int32_t next = valid ? root : INT_MAX;
@@ -826,9 +838,11 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
       if (__builtin_amdgcn_ballot_w64(len == kWalkList)) break;
     }
     if (cur >= total) break;
-    evaluate(kWalkFlush);
+    if (any_dm) evaluate(std::true_type{}, kWalkFlush);
+    else evaluate(std::false_type{}, kWalkFlush);
   }
-  evaluate(kWalkList);
+  if (any_dm) evaluate(std::true_type{}, kWalkList);
+  else evaluate(std::false_type{}, kWalkList);
   (void)h;
   double v2 = 0.0;
   if (valid) {
