@@ -538,9 +538,13 @@ __device__ __forceinline__ void upward_node(int32_t id, int32_t end, int d, RefN
   out.skip = p.skip; out.body = p.body; out.depth = p.depth; out.first = p.first;
   out.size = p.size; out.esum = p.esum;
   out.qxx = p.qxx; out.qyy = p.qyy; out.qzz = p.qzz; out.qxy = p.qxy; out.qxz = p.qxz; out.qyz = p.qyz;
+  // Upper edge of the opening test's band for r^2 (NEXT tests r^2 + 1e-20,
+  // so the 1e-20 is taken off the edge); NaN, and every decision exact, where
+  // (size/theta)^2 <= 1e-16, where the 1e-20 would eat the band's margin.
   const double q = size * inv_theta;
   const double q2 = q * q;
-  const float acc2 = (q2 > 1e-36 && q2 < 1e36) ? __double2float_ru(q2 * (1.0 + 2e-6)) : __builtin_nanf("");
+  const float acc2 =
+      (q2 > 1e-16 && q2 < 1e36) ? __double2float_ru(q2 * (1.0 + 2e-6) - 1e-20) : __builtin_nanf("");
   heads[id] = WalkHead{cx, cy, cz, acc2, M == 0.0 ? (end | kLinkZeroMass) : end};
 }
 
@@ -650,8 +654,9 @@ __device__ __forceinline__ HeadRegs load_head(const WalkHead* __restrict__ heads
 }
 
 // Lower edge of the opening test's band from its upper edge acc2 (FP32,
-// (size/theta)^2 (1 + 2e-6) rounded up): below (size/theta)^2 (1 - 1e-6).
-// Squared distances between the edges are decided by NEXT's expression.
+// (size/theta)^2 (1 + 2e-6) - 1e-20 rounded up, for (size/theta)^2 > 1e-16):
+// r^2 below it has r^2 + 1e-20 below (size/theta)^2 (1 - 2e-6).  Squared
+// distances between the edges are decided by NEXT's expression.
 constexpr double kBandLow = (1.0 - 2e-6) / (1.0 + 2.2e-6);
 
 // Accepted nodes a lane holds (a ring in LDS), and how many of each lane's
@@ -694,8 +699,8 @@ static_assert(kWalkFlush <= kWalkList, "walk ring");
 // octree.h:158-177, so the sums are NEXT's bit for bit.
 //
 // Opening test of octree.h:153, (size/dist) < theta with
-// dist = sqrt(r2 + 1e-20): decided from r2 + 1e-20 against the node's FP32
-// band edges around (size/theta)^2 (relative width about 4e-6; rounding
+// dist = sqrt(r2 + 1e-20): decided from r2 against the node's FP32 band
+// edges around (size/theta)^2 - 1e-20 (relative width about 4e-6; rounding
 // errors of either form are below 1e-15), and by NEXT's exact expression
 // inside the band, so the decision is NEXT's bit for bit and opened nodes
 // cost no sqrt or divide.
@@ -725,7 +730,7 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
   const int32_t self_link = valid ? ~(int32_t)i : INT_MAX;   // link of this lane's own leaf
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   uint32_t npp = 0, npc = 0, nop = 0, nit = 0, nev = 0;
-  int head = 0, len = 0;                                   // this lane's ring: [head, head + len)
+  int ws = 0, len = 0;                                     // this lane's ring: next free slot, entries before it
   const uint32_t zoff = (uint32_t)sc->total_nodes << 7;   // the all-zero record (k_emit)
 
   // Evaluate up to `limit` of the oldest entries (octree.h:139-176 for an
@@ -735,16 +740,17 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
   // code.
   auto evaluate = [&](auto dm, int limit) {
     const int take = len < limit ? len : limit;
+    int rs = ws - len;                                     // the oldest entry's slot
+    if (rs < 0) rs += kWalkList;
     for (int k = 0; __builtin_amdgcn_ballot_w64(k < take); ++k) {
       if (COUNT) ++nev;
-      int slot = head + k;
-      if (slot >= kWalkList) slot -= kWalkList;
       // Every lane runs every round: a lane with no entry left reads the
       // all-zero record after the last node, and so does a leaf entry for
       // its quadrupole (leaf records carry none); both add exact zeros
       // (m = 0, Q = 0, every factor finite).  Byte offsets: e << 7 drops
       // the leaf bit.
-      const int32_t e = mine[slot * kWave];
+      const int32_t e = mine[rs * kWave];
+      rs = rs == kWalkList - 1 ? 0 : rs + 1;
       const bool on = k < take;
       const uint32_t off = on ? (uint32_t)e << 7 : zoff;
       const uint32_t qoff = (on && e >= 0) ? off : zoff;
@@ -780,8 +786,6 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
       a1 = __builtin_fma(dy, c, __builtin_fma(sy, inv5, a1));
       a2 = __builtin_fma(dz, c, __builtin_fma(sz, inv5, a2));
     }
-    head += take;
-    if (head >= kWalkList) head -= kWalkList;
     len -= take;
   };
   // dark-matter targets in this wave (wave-uniform)
@@ -806,26 +810,31 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
       const HeadRegs h2 = load_head(heads, (uint32_t)sk);
       // This lane decides cur if its own path is at cur.  The decision is
       // formed without branching on it: every lane computes the test, and
-      // every lane writes cur into its ring slot head+len, which is free
-      // (each ring holds fewer than kWalkList entries inside this loop) and
-      // becomes an entry only if the lane accepts.
+      // every lane writes cur into its ring's free slot ws (each ring holds
+      // fewer than kWalkList entries inside this loop), which becomes an
+      // entry only if the lane accepts.
       const bool here = next == cur;
       const bool pass = (h.link >= 0 && (h.link & kLinkZeroMass)) || h.link == self_link;   // octree.h:134-135
       bool accept = leaf;
       if (!leaf) {                                            // wave-uniform
         const double dx = h.cx - px, dy = h.cy - py, dz = h.cz - pz;
-        const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
+        // Against the band edges (for r^2, about 4e-6 wide) any rounding of
+        // r^2 decides alike: FMAs here, NEXT's expression, as NEXT rounds
+        // it, inside the band.
+        const double r2f = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
         const double hi = (double)__uint_as_float(h.acc2_bits);   // NaN: no fast decision
-        accept = r2e > hi;
-        const bool band = here && !accept && !(r2e < hi * kBandLow);
-        if (band) accept = (nodes[cur].size / sqrt(r2e)) < theta;   // rare: NEXT's exact expression
+        accept = r2f > hi;
+        const bool band = here && !accept && !(r2f < hi * kBandLow);
+        if (band) {                                           // rare: NEXT's exact expression
+          const double r2e = (dx * dx + dy * dy + dz * dz) + 1e-20;
+          accept = (nodes[cur].size / sqrt(r2e)) < theta;
+        }
       }
       const bool take = here && !pass && accept;
       const bool open = here && !pass && !accept;
-      int slot = head + len;
-      if (slot >= kWalkList) slot -= kWalkList;
-      mine[slot * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
+      mine[ws * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
       len += take ? 1 : 0;
+      ws = take ? (ws == kWalkList - 1 ? 0 : ws + 1) : ws;
       next = here ? (open ? cur + 1 : sk) : next;
       if (COUNT) {
         npp += (take && leaf) ? 1u : 0u;
