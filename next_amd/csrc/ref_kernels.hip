@@ -833,16 +833,20 @@ void k_walk(const Group* __restrict__ groups, int64_t ngroups,
       const bool take = here && !pass && accept;
       const bool open = here && !pass && !accept;
       mine[ws * kWave] = leaf ? (int32_t)(cur | 0x80000000u) : cur;
-      len += take ? 1 : 0;
-      ws = take ? (ws == kWalkList - 1 ? 0 : ws + 1) : ws;
-      next = here ? (open ? cur + 1 : sk) : next;
+      const int tk = take ? 1 : 0;
+      len += tk;
+      ws += tk;
+      if (ws == kWalkList) ws = 0;
+      // an opening lane is here (next == cur): its next node is next + 1
+      const int op = open ? 1 : 0;
+      next = (here && !open) ? sk : next + op;
       if (COUNT) {
         npp += (take && leaf) ? 1u : 0u;
         npc += (take && !leaf) ? 1u : 0u;
         nop += open ? 1u : 0u;
       }
       if (COUNT) ++nit;
-      if (__builtin_amdgcn_ballot_w64(open)) { cur = cur + 1; h = h1; }
+      if (__builtin_amdgcn_ballot_w64(op != 0)) { cur = cur + 1; h = h1; }
       else { cur = sk; h = h2; }
       if (__builtin_amdgcn_ballot_w64(len == kWalkList)) break;
     }
