@@ -474,7 +474,10 @@ def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(2
 #     passes reads keys and values (12 B) and writes them (12 B); the gather
 #     reads the permutation (4 B) and the 68-byte state and writes it (68 B).
 #     P = 3 * depth / 8 rounded up, at most 8 (the sort covers the digits
-#     the last build's depth needs);
+#     the last build's depth needs).  A build sorted by the nearly-sorted
+#     path instead reads the keys for their descents (8 B), then reads them
+#     (8 B) and writes keys and permutation (12 B); one redone after it
+#     pays for both;
 #   node layout: lcp reads key and system (12 B) and writes lcp and node
 #     count (8 B); the count's scan reads it twice and writes it (12 B); the
 #     layout kernel reads lcp, preorder offset, x y z m and system (44 B) and
@@ -486,21 +489,30 @@ def serving(nreq: int, rate: float, seed: int = 5, n_max: int = 131072, steps=(2
 HBM_FIXED_BYTES = {"kick_ms": 124, "keys_ms": 64}
 
 
-def phase_bytes(n: int, nodes: int, depth: int) -> dict:
+def phase_bytes(n: int, nodes: int, depth: int, builds: int = 0, patched: int = 0, redos: int = 0) -> dict:
     """Bytes per body of each phase (the comment above) for a tree of
-    `nodes` nodes (leaves included) and `depth` levels over n bodies."""
+    `nodes` nodes (leaves included) and `depth` levels over n bodies; the
+    sort's averaged over `builds` builds of which `patched` took the
+    nearly-sorted path and `redos` of those were redone with the radix sort
+    (builds 0: every build radix-sorted)."""
     passes = min(8, -(-3 * depth // 8)) if 0 < depth < 21 else 8
     internal = max(nodes - n, 0)
     per_int = internal / n if n else 0.0
     out = dict(HBM_FIXED_BYTES)
-    out["sort_ms"] = 8 + 24 * passes + 140
+    radix, patch = 8 + 24 * passes, 8 + 8 + 12
+    if builds > 0:
+        sort = ((builds - patched) * radix + patched * patch + redos * radix) / builds
+    else:
+        sort = radix
+    out["sort_ms"] = round(sort + 140, 1)
     out["build_ms"] = round(12 + 8 + 12 + 44 + 112 + 8 * per_int, 1)
     out["upward_ms"] = round(per_int * (8 + 4 + 144) + max(nodes - 1, 0) / n * 64, 1) if n else 0.0
     return out
 
 
 def phase_rooflines(n: int, tim: dict, k: int) -> dict:
-    per = phase_bytes(n, int(tim.get("nodes", 0)), int(tim.get("depth", 0)))
+    per = phase_bytes(n, int(tim.get("nodes", 0)), int(tim.get("depth", 0)), int(tim.get("steps", 0)),
+                      int(tim.get("sort_patched", 0)), int(tim.get("sort_redos", 0)))
     out = {}
     for key, per_body in per.items():
         if key not in tim:
@@ -670,7 +682,8 @@ def main() -> int:
                       "kernels_per_step": tg["graph_nodes"] / max(tg["graph_period"], 1),
                       "host_waits_per_step": tg["host_waits"] / args.steps, "graph_aborts": tg["graph_aborts"],
                       "steps_from_graph": tg["graph_steps"]} if tg else None),
-        "tree": {"nodes": tim["nodes"], "depth": tim["depth"]},
+        "tree": {"nodes": tim["nodes"], "depth": tim["depth"], "builds": k,
+                 "builds_nearly_sorted": tim["sort_patched"], "builds_redone_radix": tim["sort_redos"]},
         "phases_hbm": phase_rooflines(args.n, tim, k),
     }
     log("headline: %.3f ms/step (captured %s)" % (ms_per_step, "%.3f" % (captured_ms / args.steps) if tg else "-"))

@@ -86,7 +86,7 @@ typedef struct nbx_ctx nbx_ctx;
 typedef struct nbx_timing {
   int64_t steps;       /* steps (or accel evaluations) accumulated */
   double keys_ms;      /* bounding box + Morton keys */
-  double sort_ms;      /* radix sort + reorder of the state */
+  double sort_ms;      /* sort (radix, or nearly-sorted path) + reorder of the state */
   double build_ms;     /* node layout from sorted keys */
   double upward_ms;    /* multipole pass */
   double walk_ms;      /* tree walk (+ fused closing half kick) */
@@ -105,6 +105,8 @@ typedef struct nbx_timing {
   int64_t graph_period;   /* steps in the captured period */
   int64_t graph_aborts;   /* captured runs redone step by step (a build did not fit) */
   int64_t host_waits;     /* host waits on the device inside steps and runs */
+  int64_t sort_patched;   /* tree builds sorted by the nearly-sorted path (bodies in the last build's order) */
+  int64_t sort_redos;     /* of those, builds redone with the radix sort (the bodies had moved too much) */
 } nbx_timing;
 
 /* Fill *cfg with NEXT's settings: device 0, reference mode, FP64, theta 0.5,

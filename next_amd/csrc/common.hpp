@@ -46,7 +46,7 @@ static_assert(sizeof(WalkHead) == 32, "WalkHead layout");
 // Small per-context device block: flags and the per-level node counts the
 // host needs to launch the upward pass.
 struct DevScalars {
-  int32_t err;              // bit 0: non-finite position; bit 1: radix sort wait ran out
+  int32_t err;              // bit 0: non-finite position; bit 1: radix sort wait ran out; bit 2: patch sort refused
   int32_t collisions;       // adjacent equal primary keys
   int32_t coincident;       // adjacent equal primary and secondary keys
   int32_t total_nodes;              // (internal nodes: the sum of level_count)
@@ -63,7 +63,7 @@ struct DevScalars {
   int32_t abort_flags;
   int32_t level_off[kMaxDepthSlots];
   int32_t step_count;                   // next entry of the dt log of a captured run
-  int32_t pad_;
+  uint32_t ndesc;                       // descents of nearly sorted keys (patch_sort_u64), cleared per build
 };
 
 constexpr int32_t kAbortCapacity = 1;   // nodes or node-list entries beyond the buffers

@@ -98,6 +98,11 @@ struct nbx_ctx {
   const uint64_t* skey2 = nullptr;
   int64_t total_nodes = 0;
   int32_t max_depth = 0;
+  // The bodies are in the key order of the last build (nearly the order of
+  // the next one), so the next eager build of a single system may try the
+  // patch sort; sort_patched: this build did.
+  bool patch_ok = false;
+  bool sort_patched = false;
   int prof = 0;
   hipEvent_t ev[EV_COUNT] = {};
   hipEvent_t ev_scalars = nullptr;  // the build's scalar read-back has landed
@@ -303,6 +308,7 @@ nbx_status set_systems(nbx_ctx* c) {
   c->tree_valid = false;
   c->layout_valid = false;
   c->max_depth = 0;           // the next build sorts every digit
+  c->patch_ok = false;        // bodies in the order they came in
   return NBX_OK;
 }
 
@@ -318,8 +324,12 @@ int sys_bits(int nsys) {
 // bodies in different leaves of such a tree differ there; 0: all 21 digits of
 // the first word), rounded up to whole 8-bit passes; k_fix_runs then orders
 // the runs of bodies that agree on those bits by their whole keys, so the
-// order is the full keys' whatever the depth of this build.
-const uint64_t* sort_bodies(nbx_ctx* c, int digits) {
+// order is the full keys' whatever the depth of this build.  patch (a single
+// system whose bodies are in the last build's key order): patch_sort_u64
+// instead of the radix passes, which sorts all 63 bits; when it refuses
+// (kSortErrPatch, seen with the node counts) the build is redone with the
+// radix sort from the same state (build_tree).
+const uint64_t* sort_bodies(nbx_ctx* c, int digits, bool patch) {
   const int64_t n = c->n;
   const int S = (int)c->sys_n.size();
   hipStream_t s = c->stream;
@@ -329,10 +339,16 @@ const uint64_t* sort_bodies(nbx_ctx* c, int digits) {
   launch_keys(n, St, sy, P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), c->sc, s);
   record(c, EV_KEYS);
   uint32_t* scr = P<uint32_t>(c->scratch);
+  c->sort_patched = patch && S == 1;
   const int passes = digits > 0 && digits < kKeyDigits ? (3 * digits + 7) / 8 : 8;
-  const int lo = std::max(0, 63 - 8 * passes);
-  const bool in1 = radix_sort_u64(P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key[1]),
-                                  P<uint32_t>(c->idx[1]), n, lo, 63 - lo, scr, &c->sc->err, s);
+  const int lo = c->sort_patched ? 0 : std::max(0, 63 - 8 * passes);
+  bool in1 = true;
+  if (c->sort_patched)
+    patch_sort_u64(P<uint64_t>(c->key[0]), P<uint64_t>(c->key[1]), P<uint32_t>(c->idx[1]), n, &c->sc->ndesc, scr,
+                   &c->sc->err, s);
+  else
+    in1 = radix_sort_u64(P<uint64_t>(c->key[0]), P<uint32_t>(c->idx[0]), P<uint64_t>(c->key[1]),
+                         P<uint32_t>(c->idx[1]), n, lo, 63 - lo, scr, &c->sc->err, s);
   uint64_t* skey = P<uint64_t>(c->key[in1 ? 1 : 0]);
   const uint32_t* perm = P<uint32_t>(c->idx[in1 ? 1 : 0]);
   if (S > 1) {
@@ -366,7 +382,7 @@ const uint64_t* sort_bodies(nbx_ctx* c, int digits) {
 nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
   const int64_t n = c->n;
   hipStream_t s = c->stream;
-  const uint64_t* skey = sort_bodies(c, c->max_depth);
+  const uint64_t* skey = sort_bodies(c, c->max_depth, c->patch_ok);
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
@@ -543,7 +559,21 @@ nbx_status refit_tree(nbx_ctx* c) {
 nbx_status build_tree(nbx_ctx* c) {
   const uint64_t* skey = nullptr;
   NBXCHK(build_pre(c, &skey));
-  return build_post(c, skey);
+  if (c->sort_patched) {
+    // (build_post waits for the same read-back first)
+    HIPCHK(c, hipEventSynchronize(c->ev_scalars));
+    if (c->hsc->err & kSortErrPatch) {                     // the bodies moved too much: radix sort
+      c->cur = 1 - c->cur;                                 // the state before the gather
+      if (c->amag_live) c->acur = 1 - c->acur;
+      c->patch_ok = false;
+      ++c->tim.sort_redos;
+      NBXCHK(build_pre(c, &skey));
+    }
+    ++c->tim.sort_patched;
+  }
+  const nbx_status r = build_post(c, skey);
+  c->patch_ok = r == NBX_OK && c->sys_n.size() == 1;
+  return r;
 }
 
 // Grid of the physics walk: the group count if the host has it, else its bound.
@@ -702,7 +732,7 @@ nbx_status prepare_bounded(nbx_ctx* c, int32_t nsteps) {
 void build_bounded(nbx_ctx* c) {
   const int64_t n = c->n;
   hipStream_t s = c->stream;
-  const uint64_t* skey = sort_bodies(c, c->max_depth);   // the depth when the run was captured
+  const uint64_t* skey = sort_bodies(c, c->max_depth, false);   // the depth when the run was captured
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);

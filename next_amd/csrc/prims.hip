@@ -346,6 +346,192 @@ bool radix_sort_impl(K* k0, uint32_t* v0, K* k1, uint32_t* v1, int64_t n, int lo
   return (passes & 1) != 0;
 }
 
+// ---- nearly sorted keys (patch_sort_u64) ----
+// Scratch words: [0] taken positions m, [1] refused, then the descent list,
+// the taken positions in order, their keys in order (8-byte words) and the
+// positions those keys came from.
+constexpr int kPatchBlock = 1024;
+constexpr int kPatchTake = 2 * kPatchMaxDesc;         // taken positions at most
+constexpr size_t kPatchDesc = 16;
+constexpr size_t kPatchM = kPatchDesc + kPatchMaxDesc;
+constexpr size_t kPatchDk = kPatchM + kPatchTake;     // even: 8-byte aligned
+constexpr size_t kPatchDp = kPatchDk + 2 * kPatchTake;
+static_assert(kPatchDp + kPatchTake == patch_scratch_words(), "patch scratch layout");
+static_assert(kPatchDk % 2 == 0, "patch key alignment");
+
+__global__ __launch_bounds__(kScanBlock) void k_patch_desc(const uint64_t* __restrict__ k, int64_t n,
+                                                           uint32_t* __restrict__ ndesc, uint32_t* __restrict__ desc) {
+  const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  if (i + 1 < n && k[i] > k[i + 1]) {
+    const uint32_t j = atomicAdd(ndesc, 1u);
+    if (j < (uint32_t)kPatchMaxDesc) desc[j] = (uint32_t)i;
+  }
+}
+
+// Bitonic sort of len (a power of two, at most kPatchTake) entries in LDS by
+// (a, b), all threads of the block taking part.
+template <typename A>
+__device__ void lds_bitonic(A* a, uint32_t* b, int len) {
+  for (int kk = 2; kk <= len; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < len; i += kPatchBlock) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & kk) == 0;
+          const bool gt = a[i] > a[l] || (a[i] == a[l] && b[i] > b[l]);
+          if (gt == up) {
+            const A ta = a[i]; a[i] = a[l]; a[l] = ta;
+            const uint32_t tb = b[i]; b[i] = b[l]; b[l] = tb;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// One workgroup: the taken positions (both ends of every descent, in order,
+// each once), the check that the untaken keys are in order, and the taken
+// keys sorted with their positions.
+__global__ __launch_bounds__(kPatchBlock) void k_patch_plan(const uint64_t* __restrict__ k, int64_t n,
+                                                            const uint32_t* __restrict__ ndesc,
+                                                            uint32_t* __restrict__ plan, int32_t* __restrict__ err) {
+  __shared__ uint32_t pos[kPatchTake];
+  __shared__ uint32_t aux[kPatchTake];
+  __shared__ uint64_t key[kPatchTake];
+  __shared__ uint32_t scr[kPatchBlock / kWave + 1];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  const uint32_t nd = *ndesc;
+  if (t == 0) bad = nd > (uint32_t)kPatchMaxDesc;
+  __syncthreads();
+  if (bad) {
+    if (t == 0) { plan[0] = 0; plan[1] = 1; atomicOr(err, kSortErrPatch); }
+    return;
+  }
+  int len = 1;
+  while (len < 2 * (int)nd) len <<= 1;
+  const uint32_t* desc = plan + kPatchDesc;
+  for (int i = t; i < len; i += kPatchBlock) {
+    pos[i] = i < 2 * (int)nd ? desc[i >> 1] + (uint32_t)(i & 1) : 0xffffffffu;
+    aux[i] = 0;
+  }
+  __syncthreads();
+  lds_bitonic(pos, aux, len);
+  // each position once (the ends of neighbouring descents coincide)
+  const int per = (len + kPatchBlock - 1) / kPatchBlock;      // at most 2
+  uint32_t f[2] = {0, 0};
+  for (int q = 0; q < per; ++q) {
+    const int i = t * per + q;
+    if (i < len) f[q] = pos[i] != 0xffffffffu && (i == 0 || pos[i] != pos[i - 1]);
+  }
+  uint32_t tot;
+  const uint32_t base = block_excl_scan_u32<kPatchBlock>(f[0] + f[1], scr, &tot);
+  uint32_t m_pos[2];
+  for (int q = 0; q < per; ++q) m_pos[q] = t * per + q < len ? pos[t * per + q] : 0u;
+  __syncthreads();
+  {
+    uint32_t o = base;
+    for (int q = 0; q < per; ++q)
+      if (f[q]) aux[o++] = m_pos[q];
+  }
+  __syncthreads();
+  const int m = (int)tot;
+  // across each run of taken positions [a, b]: k[a - 1] <= k[b + 1]
+  for (int i = t; i < m; i += kPatchBlock) {
+    const uint32_t a = aux[i];
+    if (i > 0 && aux[i - 1] + 1 == a) continue;              // not a run start
+    int j = i;
+    while (j + 1 < m && aux[j + 1] == aux[j] + 1) ++j;
+    const uint32_t b = aux[j];
+    if (a > 0 && (int64_t)b + 1 < n && k[a - 1] > k[b + 1]) bad = 1;
+  }
+  __syncthreads();
+  if (bad) {
+    if (t == 0) { plan[0] = 0; plan[1] = 1; atomicOr(err, kSortErrPatch); }
+    return;
+  }
+  uint32_t* M = plan + kPatchM;
+  int lenm = 1;
+  while (lenm < m) lenm <<= 1;
+  for (int i = t; i < lenm; i += kPatchBlock) {
+    if (i < m) {
+      M[i] = aux[i];
+      key[i] = k[aux[i]];
+      pos[i] = aux[i];
+    } else {
+      key[i] = ~0ull;
+      pos[i] = 0xffffffffu;
+    }
+  }
+  __syncthreads();
+  lds_bitonic(key, pos, lenm);
+  uint64_t* Dk = reinterpret_cast<uint64_t*>(plan + kPatchDk);
+  uint32_t* Dp = plan + kPatchDp;
+  for (int i = t; i < m; i += kPatchBlock) { Dk[i] = key[i]; Dp[i] = pos[i]; }
+  if (t == 0) { plan[0] = (uint32_t)m; plan[1] = 0; }
+}
+
+// Positions taken below p: the taken positions are M[0, m), in order.
+__device__ __forceinline__ int count_below(const uint32_t* __restrict__ M, int m, uint32_t p) {
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (M[mid] < p) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Threads [0, n): the untaken keys, each to (untaken keys before it) +
+// (taken keys below it).  Threads [n, n + m): taken key r to r + (untaken
+// keys at most it), found by a binary search over positions in which a taken
+// position stands for the nearest untaken one below it.
+__global__ __launch_bounds__(kScanBlock) void k_patch_apply(const uint64_t* __restrict__ k, int64_t n,
+                                                            const uint32_t* __restrict__ plan,
+                                                            uint64_t* __restrict__ kout, uint32_t* __restrict__ perm) {
+  const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  const bool refused = plan[1] != 0;
+  const int m = refused ? 0 : (int)plan[0];
+  const uint32_t* M = plan + kPatchM;
+  const uint64_t* Dk = reinterpret_cast<const uint64_t*>(plan + kPatchDk);
+  const uint32_t* Dp = plan + kPatchDp;
+  if (i < n) {
+    if (refused) { kout[i] = k[i]; perm[i] = (uint32_t)i; return; }
+    const int below = count_below(M, m, (uint32_t)i);
+    if (below < m && M[below] == (uint32_t)i) return;        // taken: placed by its own thread
+    const uint64_t key = k[i];
+    int lo = 0, hi = m;                                      // taken keys below key
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (Dk[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    const int64_t o = i - below + lo;
+    kout[o] = key;
+    perm[o] = (uint32_t)i;
+    return;
+  }
+  const int64_t r = i - n;
+  if (r >= m) return;
+  const uint64_t key = Dk[r];
+  // nearest untaken position at or below p (-1: none)
+  auto untaken_at_or_below = [&](int64_t p) {
+    int j = count_below(M, m, (uint32_t)(p + 1)) - 1;        // last taken position <= p
+    while (j >= 0 && (int64_t)M[j] == p) { --p; --j; }
+    return p;
+  };
+  int64_t lo = -1, hi = n;                                   // f(lo) true, f(hi) false
+  while (hi - lo > 1) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    const int64_t u = untaken_at_or_below(mid);
+    if (u < 0 || k[u] <= key) lo = mid; else hi = mid;
+  }
+  const int64_t u = lo < 0 ? -1 : untaken_at_or_below(lo);
+  const int64_t cnt = u < 0 ? 0 : u + 1 - count_below(M, m, (uint32_t)u);
+  const int64_t o = r + cnt;
+  kout[o] = key;
+  perm[o] = Dp[r];
+}
+
 }  // namespace
 
 void scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uint32_t* scratch, hipStream_t s) {
@@ -366,6 +552,16 @@ bool radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int6
 bool radix_sort_u32(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int lo, int bits,
                     uint32_t* scratch, int32_t* err, hipStream_t s) {
   return radix_sort_impl<uint32_t>(k0, v0, k1, v1, n, lo, bits, scratch, err, s);
+}
+
+void patch_sort_u64(const uint64_t* k, uint64_t* kout, uint32_t* perm, int64_t n, uint32_t* ndesc,
+                    uint32_t* scratch, int32_t* err, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_patch_desc, dim3((unsigned)div_up(n, kScanBlock)), dim3(kScanBlock), 0, s, k, n, ndesc,
+                     scratch + kPatchDesc);
+  hipLaunchKernelGGL(k_patch_plan, dim3(1), dim3(kPatchBlock), 0, s, k, n, ndesc, scratch, err);
+  hipLaunchKernelGGL(k_patch_apply, dim3((unsigned)div_up(n + kPatchTake, kScanBlock)), dim3(kScanBlock), 0, s, k, n,
+                     scratch, kout, perm);
 }
 
 }  // namespace nbx

@@ -51,9 +51,26 @@ void scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total, uin
 // loop; if a wait runs out, bit kSortErrTimeout is set in *err (a device
 // word, may be null) and the order is not valid.
 constexpr int32_t kSortErrTimeout = 2;
+constexpr int32_t kSortErrPatch = 4;
 bool radix_sort_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, int64_t n, int lo, int bits,
                     uint32_t* scratch, int32_t* err, hipStream_t s);
 bool radix_sort_u32(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int lo, int bits,
+                    uint32_t* scratch, int32_t* err, hipStream_t s);
+
+// Sort of keys that are nearly sorted already (the order of the last sort,
+// a few keys moved): k (n keys) into kout, with perm[j] the position in k of
+// kout[j].  The descents k[i] > k[i+1] are collected (*ndesc, a device word,
+// must be zero on entry); both ends of every descent are taken out, and the
+// rest is in order when, across each run of taken positions, the key before
+// it is at most the key after it.  The taken keys (at most 2 kPatchMaxDesc)
+// are sorted in one workgroup and merged with the rest by binary search:
+// keys in ascending order, equal keys in no particular order.  With more
+// than kPatchMaxDesc descents, or the rest out of order, bit kSortErrPatch
+// is set in *err, kout is a copy of k and perm the identity: the caller sorts
+// again with radix_sort_u64.  Scratch: patch_scratch_words() words.
+constexpr int kPatchMaxDesc = 1024;
+constexpr size_t patch_scratch_words() { return 16 + 9 * (size_t)kPatchMaxDesc; }
+void patch_sort_u64(const uint64_t* k, uint64_t* kout, uint32_t* perm, int64_t n, uint32_t* ndesc,
                     uint32_t* scratch, int32_t* err, hipStream_t s);
 
 }  // namespace nbx

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_scalars(DevScalars* sc, int wha
   const int t = threadIdx.x;
   if (what & 1) {
     if (t < kMaxDepthSlots) sc->level_count[t] = 0;
-    if (t == 0) { sc->collisions = 0; sc->coincident = 0; sc->max_leaf_depth = 0; }
+    if (t == 0) { sc->collisions = 0; sc->coincident = 0; sc->max_leaf_depth = 0; sc->ndesc = 0; }
   }
   if ((what & 4) && t == 0) sc->err = 0;
   if ((what & 8) && t == 0) { sc->n_pp = 0ull; sc->n_pc = 0ull; sc->n_open = 0ull; sc->n_iter = 0ull; sc->n_eval = 0ull; }
@@ -110,7 +110,9 @@ __global__ __launch_bounds__(kWave) void k_bbox_final(const double* __restrict__
   if (clear && s == 0) {     // a tree build's counts and flags (k_fill_scalars, what = 1 | 4)
     const int t = threadIdx.x;
     clear->level_count[t] = 0;
-    if (t == 0) { clear->collisions = 0; clear->coincident = 0; clear->max_leaf_depth = 0; clear->err = 0; }
+    if (t == 0) {
+      clear->collisions = 0; clear->coincident = 0; clear->max_leaf_depth = 0; clear->err = 0; clear->ndesc = 0;
+    }
   }
   if (s >= sy.nsys) return;
   double lo[3] = {1e30, 1e30, 1e30}, hi[3] = {-1e30, -1e30, -1e30};

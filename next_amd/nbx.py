@@ -53,7 +53,7 @@ class Timing(C.Structure):
                 ("n_pc", C.c_int64), ("n_open", C.c_int64), ("n_iter", C.c_int64),
                 ("n_eval", C.c_int64), ("graph_steps", C.c_int64), ("graph_launches", C.c_int64),
                 ("graph_nodes", C.c_int64), ("graph_period", C.c_int64), ("graph_aborts", C.c_int64),
-                ("host_waits", C.c_int64)]
+                ("host_waits", C.c_int64), ("sort_patched", C.c_int64), ("sort_redos", C.c_int64)]
 
 
 class Request(C.Structure):

@@ -78,6 +78,10 @@ def test_phase_bytes_follow_the_tree():
     b16 = bench.phase_bytes(n, 1549025, 16)
     b20 = bench.phase_bytes(n, 1549025, 20)
     assert b16["sort_ms"] == 8 + 24 * 6 + 140 and b20["sort_ms"] == 8 + 24 * 8 + 140
+    # builds sorted by the nearly-sorted path; one of ten redone by the radix sort
+    assert bench.phase_bytes(n, 1549025, 16, 10, 10, 0)["sort_ms"] == 8 + 8 + 12 + 140
+    assert bench.phase_bytes(n, 1549025, 16, 10, 10, 1)["sort_ms"] == pytest.approx(28 + 15.2 + 140, abs=0.05)
+    assert bench.phase_bytes(n, 1549025, 16, 10, 4, 0)["sort_ms"] == pytest.approx(0.6 * 152 + 0.4 * 28 + 140, abs=0.05)
     per_int = (1549025 - n) / n
     assert b16["build_ms"] == pytest.approx(188 + 8 * per_int, abs=0.1)
     assert b16["upward_ms"] == pytest.approx(per_int * 156 + (1549025 - 1) / n * 64, abs=0.1)

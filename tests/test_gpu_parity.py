@@ -173,8 +173,43 @@ def test_tree_tracks_a_deepening_collapse():
             for k in ("geo", "mass", "com", "q"):
                 assert np.array_equal(t[k], o[k]), (steps, k)
             depths.append(int(t["depth"].max()))
-    print("tree depths", depths)
+        tm = e.timing()
+    print("tree depths", depths, "builds sorted by the nearly-sorted path", tm["sort_patched"],
+          "redone", tm["sort_redos"])
     assert depths[-1] >= depths[0] + 2      # the tree deepened: the runs of the truncated sorts were fixed
+    # after the first build the bodies stay in (nearly) key order: those
+    # builds take the nearly-sorted path, and the trees above came from it
+    assert tm["sort_patched"] >= 30
+
+
+def test_sort_redone_when_the_bodies_moved_too_much():
+    """Three clusters of light bodies (negligible forces); in one step the
+    first two trade places, so in the last build's order the new keys have a
+    few descents but the rest is out of order.  The nearly-sorted path must
+    refuse it, the build must be redone with the radix sort, and the tree
+    must be the oracle's, bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    k = 3000
+    ball = rng.normal(size=(3 * k, 3)) * 0.05
+    ctr = np.repeat(np.array([[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.3]]), k, axis=0)
+    dt = 1e-3
+    vel = np.repeat(np.array([[2.0 / dt, 0.0, 0.0], [-2.0 / dt, 0.0, 0.0], [0.0, 0.0, 0.0]]), k, axis=0)
+    cols = np.column_stack([ball + ctr, vel, np.full(3 * k, 1e-12)])
+    p = ic.Particles.from_columns(cols, np.zeros(3 * k, np.int32))
+    with Engine(theta=0.5, adaptive_dt=False) as e:
+        e.upload(p)
+        e.step(dt)                                   # first tree (radix sort), then the trade of places
+        tm = e.timing()
+        x, y, z, vx, vy, vz = e.download()
+        t = e.debug_tree()
+    assert abs(float(np.mean(x[:k])) - 1.0) < 0.05 and abs(float(np.mean(x[k:2 * k])) + 1.0) < 0.05
+    print("patched", tm["sort_patched"], "redone", tm["sort_redos"])
+    assert tm["sort_redos"] >= 1
+    o = orc.tree(x, y, z, p.m)
+    for key in ("depth", "leaf", "body", "cmask", "perm"):
+        assert np.array_equal(t[key], o[key]), key
+    for key in ("geo", "mass", "com", "q"):
+        assert np.array_equal(t[key], o[key]), key
 
 
 def test_million_body_keys_and_sampled_accel():
