@@ -103,6 +103,11 @@ struct nbx_ctx {
   // patch sort; sort_patched: this build did.
   bool patch_ok = false;
   bool sort_patched = false;
+  // After a refusal the next patch_wait builds sort in full; the wait
+  // doubles with each refusal in a row (up to 64 builds), so bodies that
+  // keep moving too much (a Plummer sphere: about 1 % of them out of order
+  // per step) rarely pay for an attempt.
+  int32_t patch_wait = 0, patch_backoff = 1;
   int prof = 0;
   hipEvent_t ev[EV_COUNT] = {};
   hipEvent_t ev_scalars = nullptr;  // the build's scalar read-back has landed
@@ -309,6 +314,8 @@ nbx_status set_systems(nbx_ctx* c) {
   c->layout_valid = false;
   c->max_depth = 0;           // the next build sorts every digit
   c->patch_ok = false;        // bodies in the order they came in
+  c->patch_wait = 0;
+  c->patch_backoff = 1;
   return NBX_OK;
 }
 
@@ -379,10 +386,10 @@ const uint64_t* sort_bodies(nbx_ctx* c, int digits, bool patch) {
   return skey;
 }
 
-nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out) {
+nbx_status build_pre(nbx_ctx* c, const uint64_t** skey_out, bool patch) {
   const int64_t n = c->n;
   hipStream_t s = c->stream;
-  const uint64_t* skey = sort_bodies(c, c->max_depth, c->patch_ok);
+  const uint64_t* skey = sort_bodies(c, c->max_depth, patch);
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
@@ -558,18 +565,24 @@ nbx_status refit_tree(nbx_ctx* c) {
 // layout, multipoles.
 nbx_status build_tree(nbx_ctx* c) {
   const uint64_t* skey = nullptr;
-  NBXCHK(build_pre(c, &skey));
+  const bool patch = c->patch_ok && c->patch_wait == 0;
+  if (c->patch_wait > 0) --c->patch_wait;
+  NBXCHK(build_pre(c, &skey, patch));
   if (c->sort_patched) {
     // (build_post waits for the same read-back first)
     HIPCHK(c, hipEventSynchronize(c->ev_scalars));
+    ++c->tim.sort_patched;
     if (c->hsc->err & kSortErrPatch) {                     // the bodies moved too much: radix sort
       c->cur = 1 - c->cur;                                 // the state before the gather
       if (c->amag_live) c->acur = 1 - c->acur;
       c->patch_ok = false;
+      c->patch_backoff = std::min(2 * c->patch_backoff, 64);
+      c->patch_wait = c->patch_backoff;
       ++c->tim.sort_redos;
-      NBXCHK(build_pre(c, &skey));
+      NBXCHK(build_pre(c, &skey, false));
+    } else {
+      c->patch_backoff = 1;
     }
-    ++c->tim.sort_patched;
   }
   const nbx_status r = build_post(c, skey);
   c->patch_ok = r == NBX_OK && c->sys_n.size() == 1;
