@@ -473,7 +473,7 @@ __global__ __launch_bounds__(kPatchBlock) void k_patch_plan(const uint64_t* __re
 }
 
 // Positions taken below p: the taken positions are M[0, m), in order.
-__device__ __forceinline__ int count_below(const uint32_t* __restrict__ M, int m, uint32_t p) {
+__device__ __forceinline__ int count_below(const uint32_t* M, int m, uint32_t p) {
   int lo = 0, hi = m;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -482,6 +482,20 @@ __device__ __forceinline__ int count_below(const uint32_t* __restrict__ M, int m
   return lo;
 }
 
+// Taken keys below key: the taken keys are Dk[0, m), in order.
+__device__ __forceinline__ int keys_below(const uint64_t* Dk, int m, uint64_t key) {
+  int lo = 0, hi = m;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (Dk[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Taken entries a workgroup copies to LDS for its searches (more: the
+// searches read the plan in global memory).
+constexpr int kPatchLds = 512;
+
 // Threads [0, n): the untaken keys, each to (untaken keys before it) +
 // (taken keys below it).  Threads [n, n + m): taken key r to r + (untaken
 // keys at most it), found by a binary search over positions in which a taken
@@ -489,22 +503,26 @@ __device__ __forceinline__ int count_below(const uint32_t* __restrict__ M, int m
 __global__ __launch_bounds__(kScanBlock) void k_patch_apply(const uint64_t* __restrict__ k, int64_t n,
                                                             const uint32_t* __restrict__ plan,
                                                             uint64_t* __restrict__ kout, uint32_t* __restrict__ perm) {
+  __shared__ uint32_t sM[kPatchLds];
+  __shared__ uint64_t sD[kPatchLds];
   const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
   const bool refused = plan[1] != 0;
   const int m = refused ? 0 : (int)plan[0];
   const uint32_t* M = plan + kPatchM;
   const uint64_t* Dk = reinterpret_cast<const uint64_t*>(plan + kPatchDk);
   const uint32_t* Dp = plan + kPatchDp;
+  const bool lds = m <= kPatchLds;                          // uniform
+  if (lds && (int64_t)blockIdx.x * kScanBlock < n) {
+    for (int j = threadIdx.x; j < m; j += kScanBlock) { sM[j] = M[j]; sD[j] = Dk[j]; }
+    __syncthreads();
+  }
   if (i < n) {
     if (refused) { kout[i] = k[i]; perm[i] = (uint32_t)i; return; }
-    const int below = count_below(M, m, (uint32_t)i);
-    if (below < m && M[below] == (uint32_t)i) return;        // taken: placed by its own thread
+    const int below = lds ? count_below(sM, m, (uint32_t)i) : count_below(M, m, (uint32_t)i);
+    const uint32_t* Mb = lds ? sM : M;
+    if (below < m && Mb[below] == (uint32_t)i) return;       // taken: placed by its own thread
     const uint64_t key = k[i];
-    int lo = 0, hi = m;                                      // taken keys below key
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (Dk[mid] < key) lo = mid + 1; else hi = mid;
-    }
+    const int lo = lds ? keys_below(sD, m, key) : keys_below(Dk, m, key);
     const int64_t o = i - below + lo;
     kout[o] = key;
     perm[o] = (uint32_t)i;
