@@ -71,6 +71,7 @@ constexpr int32_t kAbortDepth = 2;      // deeper than the levels launched
 constexpr int32_t kAbortCollision = 4;  // bodies sharing all 42 digits (the eager build reports it)
 constexpr int32_t kAbortNonFinite = 8;  // a position is NaN or infinite
 constexpr int32_t kAbortSort = 16;      // the radix sort's wait for earlier tiles ran out
+constexpr int32_t kAbortPatch = 32;     // the nearly-sorted path refused the keys (the bodies moved too much)
 
 // Per-system arrays: a context holds one or more independent particle
 // systems, stored one after another in key order.

@@ -108,6 +108,7 @@ struct nbx_ctx {
   // keep moving too much (a Plummer sphere: about 1 % of them out of order
   // per step) rarely pay for an attempt.
   int32_t patch_wait = 0, patch_backoff = 1;
+  int32_t patch_streak = 0;   // eager builds in a row that the path sorted
   int prof = 0;
   hipEvent_t ev[EV_COUNT] = {};
   hipEvent_t ev_scalars = nullptr;  // the build's scalar read-back has landed
@@ -128,12 +129,18 @@ struct nbx_ctx {
     int64_t cap_records, cap_list, groups_bound;
     int dmax;
     uint64_t grids;             // hash of the per-depth upward grids and the pair grid
+    int patch;                  // builds sorted by the nearly-sorted path
     bool operator==(const GraphKey& o) const {
       return gen == o.gen && base_dt == o.base_dt && adaptive == o.adaptive && period == o.period &&
              cap_records == o.cap_records && cap_list == o.cap_list && groups_bound == o.groups_bound &&
-             dmax == o.dmax && grids == o.grids;
+             dmax == o.dmax && grids == o.grids && patch == o.patch;
     }
   } gkey{};
+  // Captured builds sort by the nearly-sorted path (bounded_patch) when the
+  // last two eager builds did and graph_patch holds: a run in which it
+  // refused the keys aborts (kAbortPatch) and is redone step by step, and
+  // later captures use the radix sort.
+  bool graph_patch = true, bounded_patch = false;
   int64_t gnodes = 0;         // kernels in the captured period
   // sizes of the bounded build (launch_check's limits and the fixed grids)
   int64_t b_cap_records = 0, b_cap_list = 0;
@@ -316,6 +323,8 @@ nbx_status set_systems(nbx_ctx* c) {
   c->patch_ok = false;        // bodies in the order they came in
   c->patch_wait = 0;
   c->patch_backoff = 1;
+  c->patch_streak = 0;
+  c->graph_patch = true;
   return NBX_OK;
 }
 
@@ -579,10 +588,14 @@ nbx_status build_tree(nbx_ctx* c) {
       c->patch_backoff = std::min(2 * c->patch_backoff, 64);
       c->patch_wait = c->patch_backoff;
       ++c->tim.sort_redos;
+      c->patch_streak = 0;
       NBXCHK(build_pre(c, &skey, false));
     } else {
       c->patch_backoff = 1;
+      ++c->patch_streak;
     }
+  } else {
+    c->patch_streak = 0;
   }
   const nbx_status r = build_post(c, skey);
   c->patch_ok = r == NBX_OK && c->sys_n.size() == 1;
@@ -745,7 +758,7 @@ nbx_status prepare_bounded(nbx_ctx* c, int32_t nsteps) {
 void build_bounded(nbx_ctx* c) {
   const int64_t n = c->n;
   hipStream_t s = c->stream;
-  const uint64_t* skey = sort_bodies(c, c->max_depth, false);   // the depth when the run was captured
+  const uint64_t* skey = sort_bodies(c, c->max_depth, c->bounded_patch);   // the depth when the run was captured
   SysArrays sy = sysa(c);
   StateArrays St = view(c, c->cur);
   uint32_t* scr = P<uint32_t>(c->scratch);
@@ -1395,8 +1408,9 @@ nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) 
       NBXCHK(snapshot(c, &sn));
       uint64_t grids = 1469598103934665603ull ^ c->b_grid_pair;   // FNV-1a over the grids
       for (unsigned g : c->b_grid_up) grids = (grids ^ g) * 1099511628211ull;
+      c->bounded_patch = c->graph_patch && c->patch_ok && c->patch_streak >= 2 && c->sys_n.size() == 1;
       const nbx_ctx::GraphKey key{c->gen,        base_dt,         c->cfg.adaptive_dt, period, c->b_cap_records,
-                                  c->b_cap_list, c->groups_bound, c->b_dmax,          grids};
+                                  c->b_cap_list, c->groups_bound, c->b_dmax,          grids, c->bounded_patch};
       if (!c->gexec || !(key == c->gkey)) {
         NBXCHK(capture(c, base_dt, period));
         c->gkey = key;
@@ -1406,6 +1420,7 @@ nbx_status nbx_run(nbx_ctx* c, double base_dt, int32_t nsteps, double* dts_out) 
       NBXCHK(sync_scalars(c));                            // the run's one wait: did every build fit?
       if (c->hsc->abort_flags) {
         ++c->tim.graph_aborts;
+        if (c->hsc->abort_flags & kAbortPatch) c->graph_patch = false;
         NBXCHK(restore(c, sn));
         launch_set_run(c->sc, 0, c->stream);              // the flag would stop the walks of the steps below
         first_dt();                                       // the speeds of the restored state

@@ -469,7 +469,8 @@ __global__ __launch_bounds__(kWave) void k_check(DevScalars* sc, int64_t cap_rec
     if (sc->max_leaf_depth - 1 > dmax) f |= kAbortDepth;
     if (sc->collisions > 0 || sc->coincident > 0) f |= kAbortCollision;
     if (sc->err & 1) f |= kAbortNonFinite;
-    if (sc->err & 2) f |= kAbortSort;
+    if (sc->err & kSortErrTimeout) f |= kAbortSort;
+    if (sc->err & kSortErrPatch) f |= kAbortPatch;
     if (f) sc->abort_flags |= f;
   }
 }

@@ -83,3 +83,32 @@ def test_capture_is_reused_across_runs():
         t = a.timing()
         assert t["graph_steps"] == 12 and t["graph_aborts"] == 0
         assert np.array_equal(a.download(), b.download())
+
+
+def test_captured_nearly_sorted_refusal_redoes_the_run():
+    """Captured builds take the nearly-sorted path once two eager builds in a
+    row have.  Three clusters of light bodies barely move for three tiny
+    steps, then two of them trade places inside a captured run: the build's
+    nearly-sorted path refuses, the run aborts and is redone step by step,
+    and the result equals the same steps run one by one."""
+    rng = np.random.Generator(np.random.PCG64(23))
+    k = 2000
+    ball = rng.normal(size=(3 * k, 3)) * 0.05
+    ctr = np.repeat(np.array([[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.3]]), k, axis=0)
+    dt = 1e-3
+    vel = np.repeat(np.array([[2.0 / dt, 0.0, 0.0], [-2.0 / dt, 0.0, 0.0], [0.0, 0.0, 0.0]]), k, axis=0)
+    p = ic.Particles.from_columns(np.column_stack([ball + ctr, vel, np.full(3 * k, 1e-12)]),
+                                  np.zeros(3 * k, np.int32))
+    out = []
+    for graph in (0, 1):
+        with Engine(theta=0.5, adaptive_dt=False) as e:
+            e.upload(p)
+            e.run(1e-9, 3)                               # one by one: the bodies stay in key order
+            e.set_graph(graph)
+            e.reset_timing()
+            e.run(dt, 2)                                 # the clusters trade places in the first step
+            out.append((e.download(), e.timing()))
+    (s0, _), (s1, t1) = out
+    print("aborts", t1["graph_aborts"], "graph steps", t1["graph_steps"], "redone sorts", t1["sort_redos"])
+    assert t1["graph_aborts"] == 1 and t1["graph_steps"] == 0
+    assert np.array_equal(s0, s1)
