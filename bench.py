@@ -219,13 +219,14 @@ def ensemble(nsys: int, steps: int, warmup: int, ref_body_steps_per_s: float = N
             "at_target_accuracy": target}
 
 
-def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 4e-4, energy_steps: int = 200,
+def ensemble_target(systems, best, theta: float = 0.8, alpha: float = 5e-4, energy_steps: int = 200,
                     ref_body_steps_per_s: float = None, group: int = 64) -> dict:
     """The same ensemble with an opening criterion whose median force error
     against the FP64 direct sum is at most 1e-4 (relative criterion alpha
     behind a geometric guard theta, groups of at most `group` targets, the
-    autotuner's leaf and list sizes; alpha was chosen for groups of 64, and
-    groups of 32 give 1.02e-4 with it): force error on 20,000 sampled bodies
+    autotuner's leaf and list sizes; alpha is the largest of 4e-4, 5e-4 and
+    6e-4 that keeps the median below 1e-4 with groups of 64: 8.1e-5,
+    9.3e-5, 1.04e-4): force error on 20,000 sampled bodies
     after one step, ms per step, per-system |dE/E| (FP64 direct-sum
     energies) over energy_steps steps, and the per-body throughput against
     NEXT's plain FP64 algorithm (reference mode) run on the same systems one
