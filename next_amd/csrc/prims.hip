@@ -22,6 +22,13 @@
 // out sets an error bit instead of hanging.  The tile is staged in LDS in
 // digit order and written out in runs, so consecutive lanes store consecutive
 // addresses.
+//
+// Nearly sorted keys (patch_sort_u64): keys that were sorted one step ago
+// and of which a few moved.  The descents are collected, both ends of each
+// are taken out, the rest is checked to be in order, the taken keys are
+// sorted in one workgroup, and every key is placed by a merge: three
+// launches, and a refusal (too many descents, or the rest out of order)
+// that the caller answers with the radix sort.
 #include <algorithm>
 
 #include "common.hpp"
